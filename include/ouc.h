@@ -1,0 +1,190 @@
+/*
+ * ouc.h -- C ABI of the MI355X elastic P/S-decoupled RTM engine.
+ *
+ * The reference (gm340/ouc) has no API: its whole path is inlined in main()
+ * (P-S-seperating.cu:1658-2493), configured by compile-time #defines
+ * (P-S-seperating.cu:14-26) and locals (P-S-seperating.cu:1685-1688,
+ * 2113-2114), and writes raw float32 image files (P-S-seperating.cu:2379-2467).
+ * Each entry point below replaces one seam of that main(); the comment on each
+ * names the reference lines it stands in for.
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross the ABI.  Every call returns an
+ *    ouc_status; ouc_last_error() gives the message of the last failure.
+ *  - Host arrays are caller owned.  Interior arrays are nx*nz floats, x-major
+ *    (index ix*nz + iz, z fastest), the order the reference writes its images in
+ *    (P-S-seperating.cu:2383-2387).
+ *  - Coordinates are interior-relative: padded = interior + npml, the
+ *    reference's convention (Sx = L + 5l, Sz = Z_receive = L at
+ *    P-S-seperating.cu:2113-2114).
+ *  - The library owns all device memory.  Calls block until results reach the
+ *    host.  One host thread per context; contexts are independent.
+ */
+#ifndef OUC_H
+#define OUC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ouc_status {
+    OUC_OK = 0,
+    OUC_INVALID_ARG = 1,
+    OUC_OOM = 2,
+    OUC_DEVICE = 3,
+    OUC_NUMERIC_FALLBACK = 4, /* learned boundary diverged; CPML result returned */
+    OUC_IO = 5,
+    OUC_UNSUPPORTED = 6
+} ouc_status;
+
+/* Time-index semantics of the source-wavefield reconstruction.
+ * FAITHFUL reproduces the reference: the velocity boundary strips restored at
+ * backward step t hold V^{t+1} (saved after forward_u, P-S-seperating.cu:2264
+ * then 2272, restored at 2337), and the imaging condition correlates the source
+ * P velocity two steps ahead (d_Vp*_s after reshot_u, P-S-seperating.cu:964-967
+ * and 2361).  EXACT restores V^t and correlates V^{t+1}, which makes the
+ * reconstruction the exact time reverse of the forward run. */
+enum { OUC_SEM_FAITHFUL = 0, OUC_SEM_EXACT = 1 };
+
+/* Storage of the boundary strips (P-S-seperating.cu:1852-1893 allocate them in
+ * FP32).  FP16 stores each 64-value chunk as binary16 with a power-of-two
+ * scale; compute stays FP32. */
+enum { OUC_STRIP_FP32 = 0, OUC_STRIP_FP16 = 1 };
+
+/* Absorbing boundary.  CPML is the reference's (P-S-seperating.cu:2067-2105).
+ * LEARNED replaces most of the CPML pad by a thin band driven by a small
+ * convolutional network; LEARNED_FALLBACK does the same and falls back to CPML
+ * when the band's energy grows. */
+enum { OUC_BOUNDARY_CPML = 0, OUC_BOUNDARY_LEARNED = 1, OUC_BOUNDARY_LEARNED_FALLBACK = 2 };
+
+/* Image filter.  STENCIL is the reference's 5-point Laplace kernel
+ * (P-S-seperating.cu:1457-1469); FFT5 applies the same operator as a rocFFT
+ * R2C/C2R filter with the 5-point discrete symbol; FFT_CONT uses -|k|^2. */
+enum { OUC_LAP_STENCIL = 0, OUC_LAP_FFT5 = 1, OUC_LAP_FFT_CONT = 2 };
+
+/* Kernel structure.  PLAIN keeps the reference's kernel split (separate stress
+ * pass, velocity pass, strip copy, reconstruction pair and imaging kernels,
+ * P-S-seperating.cu:2258-2364).  FUSED runs one single-pass kernel per
+ * wavefield per step with strip I/O and the imaging condition folded in. */
+enum { OUC_ENGINE_PLAIN = 0, OUC_ENGINE_FUSED = 1 };
+
+/* Run parameters.  Replaces the #defines L, N, NT, Xn, Zn, R
+ * (P-S-seperating.cu:15-23) and the locals dt, dx, dz
+ * (P-S-seperating.cu:1685-1688).  Xn = nx + 2*npml, Zn = nz + 2*npml. */
+typedef struct ouc_params {
+    int32_t nx, nz;          /* interior cells */
+    int32_t npml;            /* L: absorbing pad cells on each side */
+    int32_t half_order;      /* N: stencil half-order, 1..8 (reference: 6) */
+    int32_t nt;              /* time steps (reference NT = 6000) */
+    float dx, dz, dt;        /* metres, seconds; the mute and the Laplacian use dx as dh */
+    double R;                /* CPML reflection coefficient (reference 1e-3) */
+    int32_t semantics;       /* OUC_SEM_* */
+    int32_t strip_precision; /* OUC_STRIP_* */
+    int32_t boundary;        /* OUC_BOUNDARY_* */
+    int32_t laplacian;       /* OUC_LAP_* */
+    int32_t engine;          /* OUC_ENGINE_* */
+    int32_t band;            /* learned-boundary band width in cells (0: default 8) */
+} ouc_params;
+
+/* Elastic model on the interior, x-major nx*nz.  The pad is filled by edge
+ * replication inside the library, as xiangbian() does
+ * (P-S-seperating.cu:37-119).  rho is accepted and unused: the live physics has
+ * no density (P-S-seperating.cu:373-378). */
+typedef struct ouc_model {
+    const float *vp;
+    const float *vs;
+    const float *rho; /* may be NULL */
+} ouc_model;
+
+/* One shot.  Replaces Sx, Sz, Z_receive (P-S-seperating.cu:2113-2114) and the
+ * Ricker wavelet (P-S-seperating.cu:1954-1965).  Receivers are every interior
+ * column of row rec_z, as the reference records every column of its receiver
+ * row (P-S-seperating.cu:307-311) and injects the interior ones
+ * (P-S-seperating.cu:1477-1481). */
+typedef struct ouc_shot {
+    int32_t sx, sz;          /* source cell */
+    int32_t rec_z;           /* receiver row */
+    float fm;                /* Ricker peak frequency, Hz; also sets the mute's t0 */
+    const float *wavelet;    /* optional nt samples; NULL: Ricker(fm) */
+    int32_t mute;            /* 1: direct-wave mute of the records (P-S-seperating.cu:1634-1657) */
+} ouc_shot;
+
+/* Per-call timing, measured with HIP events on the stream the kernels run on. */
+typedef struct ouc_timings {
+    double total_ms;         /* whole call, device side */
+    double forward_ms;       /* forward (source) time loop */
+    double backward_ms;      /* receiver + reconstruction + imaging loop */
+    double image_ms;         /* normalisation + Laplacian + stack */
+    int64_t step_launches;   /* stencil-step kernel launches in the two loops */
+    double step_kernel_ms;   /* summed duration of the dominant step kernel */
+    int64_t step_kernel_launches;
+    double cell_updates;     /* padded-cell x step x wavefield updates performed */
+    double canonical_bytes;  /* reference two-pass compulsory bytes for the same work */
+    double own_bytes;        /* compulsory bytes of the kernels this engine ran */
+} ouc_timings;
+
+typedef struct ouc_ctx ouc_ctx;
+
+/* Build the padded model, CPML profiles and device state for one grid.
+ * Replaces the setup in main(): allocations (P-S-seperating.cu:1782-1952),
+ * model and padding (P-S-seperating.cu:1976-2065), CPML damping
+ * (P-S-seperating.cu:2067-2105).  device: HIP ordinal. */
+ouc_status ouc_create(const ouc_params *p, const ouc_model *m, int32_t device, ouc_ctx **out);
+
+/* Forward-model one shot and return its seismograms.  Replaces the forward
+ * time loop (P-S-seperating.cu:2258-2292: forward_s, forward_u with the
+ * recording at 307-311).  seis: caller-owned [6][nx][nt] floats, components
+ * vx, vz, vpx, vpz, vsx, vsz recorded after each step's velocity update at row
+ * rec_z; receiver-major like record_vx[i*NT+t].  The P and S parts come from the
+ * div/curl split (P-S-seperating.cu:291-294).  Unmuted unless shot->mute. */
+ouc_status ouc_forward_shot(ouc_ctx *c, const ouc_shot *s, float *seis);
+
+/* One RTM shot: forward run with boundary-strip saving, direct-wave mute,
+ * backward loop (receiver propagation with record injection, source
+ * reconstruction from the strips, PP/PS imaging), normalisation and Laplacian.
+ * Replaces one pass of the shot loop (P-S-seperating.cu:2108-2376).
+ * rec_in: optional observed [2][nx][nt] (vx, vz), receiver-major; NULL uses the
+ * shot's own modelled records, as the reference does (P-S-seperating.cu:2293-2310).
+ * Images are interior nx*nz x-major; any output pointer may be NULL. */
+ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in,
+                        float *img_pp, float *img_ps, float *img_pp_lap, float *img_ps_lap);
+
+/* Batched RTM over many shots on one grid: the shots run together, one launch
+ * per time step covering every shot, with the time loops captured in HIP
+ * graphs.  Returns the stack (sum over shots, in shot order) of the Laplacian
+ * images, which replaces the stacking pass (P-S-seperating.cu:2403-2443).
+ * per_shot_ms: optional nshots floats, each shot's completion time from the
+ * start of the call. */
+ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
+                         float *stack_pp_lap, float *stack_ps_lap, float *per_shot_ms);
+
+/* Profiling mode: when on, the forward loop brackets a sample of the dominant
+ * step kernel's launches with HIP events and reports their summed duration in
+ * ouc_timings.step_kernel_ms.  No counterpart in the reference, which never
+ * times itself (P-S-seperating.cu:1663). */
+ouc_status ouc_set_profiling(ouc_ctx *c, int32_t on);
+
+/* Timing of the last forward/RTM/batch call on this context. */
+ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);
+
+/* Write one shot's Laplacian images in the reference's file format and names:
+ * <dir>/image_PP_lap_<nx>_<nz>_<shot>.dat and image_PS_lap_..., raw
+ * little-endian float32, x-major interior (P-S-seperating.cu:2379-2400).
+ * shot < 0 writes the stack names All_image_PP_lap.dat / All_image_PS_lap.dat
+ * (P-S-seperating.cu:2446-2467).  Creates dir if missing. */
+ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_t shot,
+                                 const float *pp_lap, const float *ps_lap);
+
+/* Message of the last failed call on c (or of the last failed ouc_create when
+ * c is NULL).  Never NULL. */
+const char *ouc_last_error(const ouc_ctx *c);
+
+void ouc_destroy(ouc_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OUC_H */

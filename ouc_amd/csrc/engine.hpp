@@ -1,0 +1,86 @@
+// Engine context shared by the plain and fused paths.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ouc.h"
+#include "kernels_plain.hpp"
+#include "setup.hpp"
+
+namespace ouc {
+
+struct Error : std::runtime_error {
+    ouc_status code;
+    Error(ouc_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+#define OUC_HIP(x) ::ouc::hip_check((x), #x)
+
+// Owning device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t n);
+    void release();
+    float *f() const { return (float *)ptr; }
+    template <class T> T *as() const { return (T *)ptr; }
+};
+
+// One wavefield: 32 pitched arrays in a single allocation.
+struct DevWave {
+    DevBuf mem;
+    KWave k{};
+    void alloc(const Grid &g);
+    void zero(hipStream_t s, const Grid &g);
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    void init();
+    void destroy();
+};
+
+} // namespace ouc
+
+struct ouc_ctx {
+    ouc_params p{};
+    ouc::Grid g;
+    ouc::RingDesc ring{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool profiling = false;
+
+    std::vector<float> vp_pad; // host copy (mute velocity)
+    ouc::DevBuf p2, q2, ax, az;
+    ouc::DevWave F, R;
+    ouc::DevBuf vp3[3][2];
+    ouc::DevBuf npp, nps, den;
+    ouc::DevBuf rec;    // [ncomp][nt][nx]
+    ouc::DevBuf dX, dZ; // Xn
+    ouc::DevBuf counts; // nx
+    ouc::DevBuf img[4]; // pp, ps, pp_lap, ps_lap (nx*nz)
+    ouc::DevBuf strips32, strips16, stripse8;
+    int64_t strip_slots = 0, strip_chunks = 0;
+    int strip_fp16 = -1;
+    hipEvent_t ev[8] = {};
+    hipEvent_t kev[2] = {};
+    ouc_timings last{};
+
+    ouc::KGrid kgrid() const;
+    ouc::KModel kmodel() const;
+    ouc::KRing kring() const;
+    ouc::KStrips kstrips() const;
+    void ensure_strips();
+};

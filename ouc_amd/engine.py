@@ -1,0 +1,154 @@
+"""Python handle on the HIP engine (ouc_amd/libouc.so) through its C ABI.
+
+Mirrors include/ouc.h: an Engine owns one ouc_ctx (one grid and model on one
+GPU) and runs forward shots, RTM shots and batched RTM.  There is no CPU
+fallback: if the HIP library is missing, importing this module's loader raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libouc.so")
+_lib = None
+
+
+class OucError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def lib():
+    """Load libouc.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OucError(abi.OUC_UNSUPPORTED, f"HIP library {LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    fp = P(C.c_float)
+    L.ouc_create.argtypes = [P(abi.Params), P(abi.Model), C.c_int32, P(C.c_void_p)]
+    L.ouc_create.restype = C.c_int
+    L.ouc_forward_shot.argtypes = [C.c_void_p, P(abi.Shot), fp]
+    L.ouc_forward_shot.restype = C.c_int
+    L.ouc_rtm_shot.argtypes = [C.c_void_p, P(abi.Shot), fp, fp, fp, fp, fp]
+    L.ouc_rtm_shot.restype = C.c_int
+    L.ouc_rtm_batch.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, fp, fp, fp]
+    L.ouc_rtm_batch.restype = C.c_int
+    L.ouc_get_timings.argtypes = [C.c_void_p, P(abi.Timings)]
+    L.ouc_get_timings.restype = C.c_int
+    L.ouc_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.ouc_set_profiling.restype = C.c_int
+    L.ouc_write_image_files.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, fp, fp]
+    L.ouc_write_image_files.restype = C.c_int
+    L.ouc_last_error.argtypes = [C.c_void_p]
+    L.ouc_last_error.restype = C.c_char_p
+    L.ouc_destroy.argtypes = [C.c_void_p]
+    L.ouc_destroy.restype = None
+    _lib = L
+    return _lib
+
+
+def _shot_struct(shot, nt):
+    """dict(sx, sz, rec_z, fm, wavelet=None, mute=0) -> (abi.Shot, keepalive)."""
+    w = shot.get("wavelet")
+    if w is not None:
+        w = np.ascontiguousarray(w, np.float32)
+        if w.size < nt:
+            raise ValueError("wavelet shorter than nt")
+    s = abi.Shot(int(shot["sx"]), int(shot["sz"]), int(shot["rec_z"]), float(shot.get("fm", 0.0)), abi.fptr(w),
+                 int(shot.get("mute", 0)))
+    return s, w
+
+
+class Engine:
+    """One grid + model on one GPU (ouc_create / ouc_destroy)."""
+
+    def __init__(self, params, vp, vs, rho=None, device=0):
+        self.p = params
+        self.vp = np.ascontiguousarray(vp, np.float32)
+        self.vs = np.ascontiguousarray(vs, np.float32)
+        if self.vp.shape != (params.nx, params.nz) or self.vs.shape != (params.nx, params.nz):
+            raise ValueError(f"model must be ({params.nx}, {params.nz}) x-major")
+        self.rho = None if rho is None else np.ascontiguousarray(rho, np.float32)
+        self._m = abi.Model(abi.fptr(self.vp), abi.fptr(self.vs), abi.fptr(self.rho))
+        h = C.c_void_p()
+        st = lib().ouc_create(C.byref(self.p), C.byref(self._m), int(device), C.byref(h))
+        if st != abi.OUC_OK:
+            raise OucError(st, lib().ouc_last_error(None).decode())
+        self._h = h
+
+    def _check(self, st):
+        if st != abi.OUC_OK:
+            raise OucError(st, lib().ouc_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ouc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_profiling(self, on=True):
+        self._check(lib().ouc_set_profiling(self._h, int(bool(on))))
+
+    def forward_shot(self, shot):
+        """Seismograms [6, nx, nt]: vx, vz, vpx, vpz, vsx, vsz at row rec_z."""
+        s, keep = _shot_struct(shot, self.p.nt)
+        seis = np.zeros((6, self.p.nx, self.p.nt), np.float32)
+        self._check(lib().ouc_forward_shot(self._h, C.byref(s), abi.fptr(seis)))
+        return seis
+
+    def rtm_shot(self, shot, rec_in=None):
+        s, keep = _shot_struct(shot, self.p.nt)
+        rin = None if rec_in is None else np.ascontiguousarray(rec_in, np.float32)
+        outs = [np.zeros((self.p.nx, self.p.nz), np.float32) for _ in range(4)]
+        self._check(lib().ouc_rtm_shot(self._h, C.byref(s), abi.fptr(rin), *[abi.fptr(a) for a in outs]))
+        return {"img_pp": outs[0], "img_ps": outs[1], "lap_pp": outs[2], "lap_ps": outs[3]}
+
+    def rtm_batch(self, shots, per_shot=False):
+        n = len(shots)
+        arr = (abi.Shot * n)()
+        keep = []
+        for k, sh in enumerate(shots):
+            s, w = _shot_struct(sh, self.p.nt)
+            arr[k] = s
+            keep.append(w)
+        spp = np.zeros((self.p.nx, self.p.nz), np.float32)
+        sps = np.zeros_like(spp)
+        ms = np.zeros(n, np.float32)
+        self._check(lib().ouc_rtm_batch(self._h, arr, n, abi.fptr(spp), abi.fptr(sps), abi.fptr(ms)))
+        out = {"stack_pp": spp, "stack_ps": sps}
+        if per_shot:
+            out["per_shot_ms"] = ms
+        return out
+
+    def timings(self):
+        t = abi.Timings()
+        self._check(lib().ouc_get_timings(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in abi.Timings._fields_}
+
+
+def write_image_files(directory, pp_lap, ps_lap, shot):
+    """Reference file format: <dir>/image_PP_lap_<nx>_<nz>_<shot>.dat (shot < 0: the stack names)."""
+    pp = np.ascontiguousarray(pp_lap, np.float32)
+    ps = np.ascontiguousarray(ps_lap, np.float32)
+    st = lib().ouc_write_image_files(directory.encode(), pp.shape[0], pp.shape[1], int(shot), abi.fptr(pp),
+                                     abi.fptr(ps))
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
