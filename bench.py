@@ -1,0 +1,170 @@
+"""Benchmark: batched elastic RTM shots/hour at 2048 x 1024 on one MI355X.
+
+Workload: 2048 x 1024 interior at 5 m (N = 5, 30-cell CPML, 2108 x 1084 padded),
+six layers 1800-4000 m/s with a seeded 3 % random perturbation, Ricker 20 Hz,
+dt = 0.4 ms, 6000 steps per shot, 2048 surface receivers, sources at
+x = 16 + 32 s.  A "step" is one batch of --batch shots through the full RTM path
+(forward with strip saving, mute, backward with reconstruction, receiver
+propagation and imaging, Laplacian, stack).  Inputs are resident on the GPU
+before the timed region; value = shots per hour of the whole job.
+
+Also reported: Gcell-updates/s, canonical and own-bytes HBM rates, the plain
+(reference kernel structure, one shot at a time, FP32 strips) shots/hour on a
+bounded sample, the roofline of the dominant step kernel from HIP events, and
+the FP32 CPU baseline (the oracle) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0, help="shots per step (0: engine default)")
+    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--engine", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--plain-sample-shots", type=int, default=1)
+    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    from ouc_amd import abi, velocity_models as vm
+    from ouc_amd.engine import Engine, OucError
+
+    engine = abi.ENGINE_FUSED if a.engine == "fused" else abi.ENGINE_PLAIN
+    strips = abi.STRIP_FP16 if a.strips == "fp16" else abi.STRIP_FP32
+    p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=engine, strip_precision=strips)
+    batch = a.batch or (8 if engine == abi.ENGINE_FUSED else 1)
+    nsteps_total = a.warmup + a.steps
+    if nsteps_total * batch > len(shots):
+        shots = shots * ((nsteps_total * batch) // len(shots) + 1)
+    print(f"[bench] {a.engine} engine, batch {batch}, {a.steps} timed + {a.warmup} warmup steps, "
+          f"grid {p.nx}x{p.nz} nt={p.nt}", file=sys.stderr, flush=True)
+    e = Engine(p, vp, vs)
+    e.set_profiling(True)
+    k = 0
+    for w in range(a.warmup):
+        e.rtm_batch(shots[k:k + batch])
+        k += batch
+        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+    tim = []
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        e.rtm_batch(shots[k:k + batch])
+        k += batch
+        tim.append(e.timings())
+        print(f"[bench] step {s}: {tim[-1]['total_ms']:.1f} ms device", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    dev_ms = sum(t["total_ms"] for t in tim)
+    nshots = a.steps * batch
+    shots_per_hour = nshots / (dev_ms / 1e3) * 3600.0
+    cell_updates = sum(t["cell_updates"] for t in tim)
+    canonical = sum(t["canonical_bytes"] for t in tim)
+    own = sum(t["own_bytes"] for t in tim)
+    kms = sum(t["step_kernel_ms"] for t in tim)
+    klaunch = sum(t["step_kernel_launches"] for t in tim)
+
+    # dominant kernel roofline: own compulsory bytes per launch / mean launch time
+    from ouc_amd import bytes_model
+
+    per_launch_bytes, kernel_name = bytes_model.dominant_kernel_bytes(p, engine, batch)
+    mean_ms = kms / max(klaunch, 1)
+    achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if klaunch else None
+    roofline = {
+        "bound": "hbm", "kernel": kernel_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+        "bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_ms, "launches_timed": klaunch,
+    }
+    traffic = bytes_model.measured_traffic(kernel_name)
+    if traffic is not None:
+        roofline["traffic"] = traffic
+
+    plain = None
+    if not a.no_plain and engine == abi.ENGINE_FUSED:
+        pp, _, _, _ = vm.cfg_rtm_2048(nt=a.nt, engine=abi.ENGINE_PLAIN, strip_precision=abi.STRIP_FP32)
+        e.close()
+        ep = Engine(pp, vp, vs)
+        ep.rtm_shot(shots[0])  # warm
+        ms = 0.0
+        for s in range(a.plain_sample_shots):
+            ep.rtm_shot(shots[s])
+            ms += ep.timings()["total_ms"]
+        plain = {"shots_per_hour": a.plain_sample_shots / (ms / 1e3) * 3600.0,
+                 "sample": f"{a.plain_sample_shots} shot(s), reference kernel structure, FP32 strips"}
+        ep.close()
+    else:
+        e.close()
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(p, vp, vs, shots[0], a.cpu_sample_steps)
+
+    line = {
+        "metric": "RTM shots/hour at 2048x1024 (batched); Gcell-updates/s; HBM GB/s",
+        "value": shots_per_hour,
+        "unit": "shots/hour",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dev_ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered + 3% random model)",
+        "config": {"workload": "batched elastic RTM, 2048x1024 interior, 5 m, N=5, 30-cell CPML, "
+                               f"6000 steps, {batch} shots per step, {a.strips} strips, {a.engine} engine",
+                   "nx": p.nx, "nz": p.nz, "nt": p.nt, "batch": batch},
+        "gcell_updates_per_s": cell_updates / (dev_ms / 1e3) / 1e9,
+        "canonical_gbs": canonical / (dev_ms / 1e3) / 1e9,
+        "own_bytes_gbs": own / (dev_ms / 1e3) / 1e9 if own else None,
+        "wall_s": wall,
+        "plain": plain,
+        "batched_over_plain": (shots_per_hour / plain["shots_per_hour"]) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(p, vp, vs, shot, nsteps):
+    """FP32 oracle (OpenMP) on a bounded sample: one shot at the full grid but
+    `nsteps` time steps, extrapolated to 6000 steps."""
+    from oracle import pyoracle
+    from ouc_amd import abi
+
+    threads = min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(threads)
+    ps = abi.make_params(p.nx, p.nz, p.npml, p.half_order, nsteps, p.dx, p.dz, p.dt, R=p.R)
+    t0 = time.perf_counter()
+    pyoracle.rtm_shot(ps, vp, vs, shot, precision=32)
+    dt = time.perf_counter() - t0
+    per_shot_s = dt * p.nt / nsteps
+    return {"value": 3600.0 / per_shot_s, "unit": "shots/hour", "cores": threads, "kind": "port",
+            "sample": f"1 shot of {p.nx}x{p.nz} with {nsteps} of {p.nt} steps, FP32 oracle, "
+                      f"{dt:.1f} s, extrapolated linearly in steps"}
+
+
+if __name__ == "__main__":
+    main()

@@ -177,6 +177,13 @@ ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);
 ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_t shot,
                                  const float *pp_lap, const float *ps_lap);
 
+/* Scaled-FP16 strip codec on the GPU, for bit-exact testing against a host
+ * implementation: encodes n floats (host) in 64-value chunks exactly as the
+ * strip store does, returning the binary16 bits and each chunk's power-of-two
+ * exponent.  No counterpart in the reference, whose strips are FP32
+ * (P-S-seperating.cu:1852-1893). */
+ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, uint16_t *half_bits, int8_t *exponents);
+
 /* Message of the last failed call on c (or of the last failed ouc_create when
  * c is NULL).  Never NULL. */
 const char *ouc_last_error(const ouc_ctx *c);

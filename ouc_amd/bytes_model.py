@@ -1,0 +1,62 @@
+"""Algorithmic (compulsory) HBM bytes of the engines' kernels.
+
+Each FP32 operand a kernel must read or write is counted once per cell and
+launch; stencil neighbours are reuse, not traffic.  The reference's own
+two-pass form (forward_s + forward_u, P-S-seperating.cu:205-384) is the
+canonical model: 120 B per interior and 232 B per CPML cell-update.
+"""
+import os
+
+from . import abi
+
+CANONICAL_INTERIOR = 120.0
+CANONICAL_CPML = 232.0
+
+
+def _cells(p):
+    Xn, Zn = p.nx + 2 * p.npml, p.nz + 2 * p.npml
+    upd = (Xn - 2 * p.half_order) * (Zn - 2 * p.half_order)
+    interior = p.nx * p.nz
+    return Xn, Zn, upd, interior
+
+
+def plain_velocity_bytes(p):
+    """k_velocity over the update region: reads theta, omega, 4 g, q, ax, az,
+    4 F, and the 10 state arrays (Phi, Q, S with their increments); writes the
+    10 state arrays; CPML cells also read and write the 8 O arrays."""
+    Xn, Zn, upd, interior = _cells(p)
+    base = 4.0 * (6 + 3 + 4 + 10 + 10)
+    cpml = upd - interior
+    return upd * base + cpml * 4.0 * 16
+
+
+def fused_step_bytes(p, nshots):
+    """Fused forward/receiver step per shot: reads the 8 state arrays, vp^2 and
+    vs^2, writes the 8 state arrays (72 B per interior cell); CPML cells also
+    read ax, az and read/write F and O (104 B more)."""
+    Xn, Zn, upd, interior = _cells(p)
+    cpml = upd - interior
+    return nshots * (upd * 72.0 + cpml * 104.0)
+
+
+def dominant_kernel_bytes(p, engine, nshots):
+    if engine == abi.ENGINE_FUSED:
+        return fused_step_bytes(p, nshots), "k_fused_step"
+    return plain_velocity_bytes(p), "k_velocity"
+
+
+def measured_traffic(kernel_name):
+    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 PMC
+    summary (profiles/pmc_traffic.md: lines 'kernel: bytes'), or None."""
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            line = line.strip().lstrip("-* ")
+            if line.startswith(kernel_name + ":"):
+                try:
+                    return float(line.split(":", 1)[1].split()[0])
+                except ValueError:
+                    return None
+    return None

@@ -513,6 +513,32 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, floa
     return OUC_OK;
 }
 
+ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, uint16_t *half_bits, int8_t *exponents) {
+    if (!x || n < 1 || !half_bits || !exponents) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
+    try {
+        OUC_HIP(hipSetDevice(device));
+        int64_t chunks = (n + 63) / 64;
+        DevBuf din, dh, de;
+        din.alloc(n * sizeof(float));
+        dh.alloc(n * sizeof(__half));
+        de.alloc(chunks);
+        OUC_HIP(hipMemcpy(din.ptr, x, n * sizeof(float), hipMemcpyHostToDevice));
+        KStrips s{};
+        s.f16 = dh.as<__half>();
+        s.e8 = de.as<int8_t>();
+        s.ncells = n;
+        s.nchunks = chunks;
+        s.fp16 = 1;
+        k_codec_fp16<<<(unsigned)((n + 255) / 256), 256>>>(s, din.f(), n);
+        OUC_HIP(hipGetLastError());
+        OUC_HIP(hipMemcpy(half_bits, dh.ptr, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        OUC_HIP(hipMemcpy(exponents, de.ptr, chunks, hipMemcpyDeviceToHost));
+        return OUC_OK;
+    } catch (const Error &e) {
+        return fail(nullptr, e.code, e.what());
+    }
+}
+
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t) {
     if (!c || !t) return OUC_INVALID_ARG;
     *t = c->last;

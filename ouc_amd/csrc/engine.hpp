@@ -77,6 +77,8 @@ struct ouc_ctx {
     hipEvent_t ev[8] = {};
     hipEvent_t kev[2] = {};
     ouc_timings last{};
+    void *lapfft = nullptr; // rocFFT plans and buffers (lapfft.hip)
+    void *fused = nullptr;  // fused-engine state (fused.hip)
 
     ouc::KGrid kgrid() const;
     ouc::KModel kmodel() const;

@@ -346,6 +346,13 @@ __device__ __forceinline__ float strip_get(const KStrips &s, int slot, int f, lo
     return ldexpf(__half2float(s.f16[base + q]), e);
 }
 
+// The strip codec alone (slot 0, field 0 of a single-field store).
+static __global__ __launch_bounds__(256) void k_codec_fp16(KStrips s, const float *__restrict__ x, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = q < n;
+    strip_put(s, 0, 0, q, valid, valid ? x[q] : 0.f);
+}
+
 // Velocity part of slot `slot` from the assembled totals.  Launched with
 // 64-lane blocks so each wave covers one 64-cell chunk.
 static __global__ __launch_bounds__(256) void k_strip_save_v(KRing r, KStrips s, int slot, const float *__restrict__ Vx,

@@ -46,6 +46,8 @@ def lib():
     L.ouc_set_profiling.restype = C.c_int
     L.ouc_write_image_files.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, fp, fp]
     L.ouc_write_image_files.restype = C.c_int
+    L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, P(C.c_uint16), P(C.c_int8)]
+    L.ouc_strip_codec_fp16.restype = C.c_int
     L.ouc_last_error.argtypes = [C.c_void_p]
     L.ouc_last_error.restype = C.c_char_p
     L.ouc_destroy.argtypes = [C.c_void_p]
@@ -152,3 +154,15 @@ def write_image_files(directory, pp_lap, ps_lap, shot):
                                      abi.fptr(ps))
     if st != abi.OUC_OK:
         raise OucError(st, lib().ouc_last_error(None).decode())
+
+
+def strip_codec_fp16(x, device=0):
+    """GPU scaled-FP16 strip codec on a host array: (binary16 bits, chunk exponents)."""
+    x = np.ascontiguousarray(x, np.float32)
+    h = np.zeros(x.size, np.uint16)
+    e = np.zeros((x.size + 63) // 64, np.int8)
+    st = lib().ouc_strip_codec_fp16(int(device), abi.fptr(x), x.size, h.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                    e.ctypes.data_as(C.POINTER(C.c_int8)))
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
+    return h, e
