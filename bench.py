@@ -32,12 +32,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=0, help="shots per step (0: engine default)")
+    ap.add_argument("--batch", type=int, default=0, help="shots per step (0: 16 fused, 1 plain)")
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--engine", choices=["fused", "plain"], default="fused")
-    ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp32")
     ap.add_argument("--plain-sample-shots", type=int, default=1)
-    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--cpu-sample-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     return ap.parse_args()
@@ -55,7 +55,7 @@ def main():
     engine = abi.ENGINE_FUSED if a.engine == "fused" else abi.ENGINE_PLAIN
     strips = abi.STRIP_FP16 if a.strips == "fp16" else abi.STRIP_FP32
     p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=engine, strip_precision=strips)
-    batch = a.batch or (8 if engine == abi.ENGINE_FUSED else 1)
+    batch = a.batch or (16 if engine == abi.ENGINE_FUSED else 1)
     nsteps_total = a.warmup + a.steps
     if nsteps_total * batch > len(shots):
         shots = shots * ((nsteps_total * batch) // len(shots) + 1)

@@ -30,18 +30,19 @@ def plain_velocity_bytes(p):
     return upd * base + cpml * 4.0 * 16
 
 
-def fused_step_bytes(p, nshots):
-    """Fused forward/receiver step per shot: reads the 8 state arrays, vp^2 and
-    vs^2, writes the 8 state arrays (72 B per interior cell); CPML cells also
-    read ax, az and read/write F and O (104 B more)."""
-    Xn, Zn, upd, interior = _cells(p)
-    cpml = upd - interior
-    return nshots * (upd * 72.0 + cpml * 104.0)
+def fused_core_bytes(p, nshots):
+    """k_core in forward mode, per launch: every core cell
+    [L+N, Xn-L-N) x [L+N, Zn-L-N) of every shot reads the 8 state arrays
+    (V, DV, S, DS; x and z) and vp^2, vs^2, and writes the 8 state arrays:
+    18 floats = 72 B.  Halo re-reads are cache reuse, not counted."""
+    Xn, Zn = p.nx + 2 * p.npml, p.nz + 2 * p.npml
+    core = max(0, Xn - 2 * (p.npml + p.half_order)) * max(0, Zn - 2 * (p.npml + p.half_order))
+    return nshots * core * 72.0
 
 
 def dominant_kernel_bytes(p, engine, nshots):
     if engine == abi.ENGINE_FUSED:
-        return fused_step_bytes(p, nshots), "k_fused_step"
+        return fused_core_bytes(p, nshots), "k_core"
     return plain_velocity_bytes(p), "k_velocity"
 
 

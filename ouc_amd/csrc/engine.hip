@@ -403,8 +403,7 @@ ouc_status ouc_forward_shot(ouc_ctx *c, const ouc_shot *s, float *seis) {
         probe.init(g.nt, c->profiling);
         c->F.zero(c->stream, g);
         OUC_HIP(hipEventRecord(c->ev[0], c->stream));
-        if (c->p.engine == OUC_ENGINE_FUSED) fused_forward(c, s, wav, false, 6);
-        else run_forward_plain(c, s, wav, false, 6, probe);
+        run_forward_plain(c, s, wav, false, 6, probe);
         OUC_HIP(hipEventRecord(c->ev[1], c->stream));
         if (s->mute) apply_mute(c, s, 6);
         std::vector<float> tmp((size_t)6 * g.nt * g.nx);
@@ -433,6 +432,10 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        if (c->p.engine == OUC_ENGINE_FUSED && !rec_in) {
+            fused_rtm_one(c, s, img_pp, img_ps, img_pp_lap, img_ps_lap);
+            return;
+        }
         const Grid &g = c->g;
         std::vector<float> wav = shot_wavelet(c, s);
         c->ensure_strips();
@@ -441,8 +444,7 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
         probe.init(g.nt, c->profiling);
         c->F.zero(c->stream, g);
         OUC_HIP(hipEventRecord(c->ev[0], c->stream));
-        if (c->p.engine == OUC_ENGINE_FUSED) fused_forward(c, s, wav, true, 2);
-        else run_forward_plain(c, s, wav, true, 2, probe);
+        run_forward_plain(c, s, wav, true, 2, probe);
         if (rec_in) {
             std::vector<float> tr((size_t)2 * g.nt * g.nx);
             for (int q = 0; q < 2; ++q)
@@ -455,8 +457,7 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
         }
         if (s->mute) apply_mute(c, s, 2);
         OUC_HIP(hipEventRecord(c->ev[1], c->stream));
-        if (c->p.engine == OUC_ENGINE_FUSED) fused_backward(c, s, wav);
-        else run_backward_plain(c, s, wav);
+        run_backward_plain(c, s, wav);
         OUC_HIP(hipEventRecord(c->ev[2], c->stream));
         make_images(c);
         OUC_HIP(hipEventRecord(c->ev[3], c->stream));
@@ -479,7 +480,7 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
         double cpml = padded - interior;
         double cu = (double)g.nt * (2 * padded + interior);
         double canon = (double)g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
-        fill_counts(c, cu, canon, c->p.engine == OUC_ENGINE_FUSED ? fused_own_bytes(c) : 0);
+        fill_counts(c, cu, canon, 0);
     });
 }
 

@@ -1,15 +1,537 @@
+// Fused engine: batched shots, one launch per kernel per step for the whole
+// batch (grid z = shot), band/core split, stencil state double-buffered.
+//
+// Step structure (per wavefield): k_band_stress, k_core, k_band_velocity; the
+// source reconstruction is k_recon_stress_f + k_recon_velocity_f.  Strips are
+// FP32 or scaled FP16, [shot][slot][field][ring]; slot t holds V^{t+1} and the
+// stress-like fields of step t (the reference's save after forward_u,
+// P-S-seperating.cu:2264-2272), and slot nt-1 holds the last stress fields,
+// which the reference hands over with read_last_wavefiled2
+// (P-S-seperating.cu:1318-1342).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "engine.hpp"
 #include "fused.hpp"
+#include "fused_kernels.hpp"
+#include "lapfft.hpp"
 
-void fused_init(ouc_ctx *) {}
-void fused_destroy(ouc_ctx *) {}
-void fused_forward(ouc_ctx *, const ouc_shot *, const std::vector<float> &, bool, int) {
-    throw ouc::Error(OUC_UNSUPPORTED, "fused engine not built yet");
+using namespace ouc;
+
+namespace {
+
+#define OUC_FDISPATCH(NV, ...)                                                   \
+    switch (NV) {                                                                \
+    case 1: { constexpr int NN = 1; __VA_ARGS__; } break;                        \
+    case 2: { constexpr int NN = 2; __VA_ARGS__; } break;                        \
+    case 3: { constexpr int NN = 3; __VA_ARGS__; } break;                        \
+    case 4: { constexpr int NN = 4; __VA_ARGS__; } break;                        \
+    case 5: { constexpr int NN = 5; __VA_ARGS__; } break;                        \
+    case 6: { constexpr int NN = 6; __VA_ARGS__; } break;                        \
+    case 7: { constexpr int NN = 7; __VA_ARGS__; } break;                        \
+    default: { constexpr int NN = 8; __VA_ARGS__; } break;                       \
+    }
+
+constexpr int kWaveArrays = 28; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O
+
+struct FusedState {
+    int cap = 0;           // shots the buffers hold
+    long plane = 0;
+    FGrid g{};
+    DevBuf mem_s, mem_r;   // wavefields
+    FWave S{}, R{};
+    DevBuf vp[3][2];       // source P velocity levels [cap][plane]
+    DevBuf acc[3];         // npp, nps, den
+    DevBuf strips32, strips16, e8;
+    long shot_strip = 0, shot_chunks = 0, nchunks = 0;
+    int nslots = 0, fp16 = 0;
+    DevBuf rec;            // [cap][2][nt][nx]
+    DevBuf srcpos, wav, counts;
+    DevBuf tiles_s, tiles_v, tiles_rs, tiles_rv;
+    int ntiles_s = 0, ntiles_v = 0, ntiles_rs = 0, ntiles_rv = 0;
+    DevBuf img;            // [4][nx*nz] scratch for one shot's images
+};
+
+FusedState *fs(ouc_ctx *c) { return (FusedState *)c->fused; }
+
+void wave_layout(DevBuf &mem, FWave &w, long plane, int cap) {
+    mem.alloc((size_t)kWaveArrays * plane * cap * sizeof(float));
+    float *b = mem.f();
+    const size_t span = (size_t)plane * cap;
+    float **slots[kWaveArrays] = {&w.V[0][0], &w.V[0][1], &w.V[1][0], &w.V[1][1], &w.DV[0], &w.DV[1], &w.S[0],
+                                  &w.S[1],    &w.DS[0],   &w.DS[1],   &w.th,      &w.om,     &w.gxx,   &w.gxz,
+                                  &w.gzx,     &w.gzz,     &w.F[0],    &w.F[1],    &w.F[2],   &w.F[3],  &w.O[0],
+                                  &w.O[1],    &w.O[2],    &w.O[3],    &w.O[4],    &w.O[5],   &w.O[6],  &w.O[7]};
+    for (int a = 0; a < kWaveArrays; ++a) *slots[a] = b + (size_t)a * span;
 }
-void fused_backward(ouc_ctx *, const ouc_shot *, const std::vector<float> &) {
-    throw ouc::Error(OUC_UNSUPPORTED, "fused engine not built yet");
+
+FGrid make_fgrid(const ouc_ctx *c) {
+    const Grid &g = c->g;
+    FGrid f{};
+    f.Xn = g.Xn; f.Zn = g.Zn; f.P = g.P; f.L = g.L; f.N = g.N; f.nx = g.nx; f.nz = g.nz; f.nt = g.nt;
+    f.plane = (long)g.pitched();
+    f.idx = 1.0f / g.dx; f.idz = 1.0f / g.dz; f.dt = g.dt; f.dt2 = g.dt * g.dt;
+    stencil_coeffs(g.N, f.c);
+    for (int a = g.N; a < 8; ++a) f.c[a] = 0.f;
+    f.cx0 = g.L + g.N; f.cx1 = g.Xn - g.L - g.N;
+    f.cz0 = g.L + g.N; f.cz1 = g.Zn - g.L - g.N;
+    if (f.cx1 <= f.cx0 || f.cz1 <= f.cz0) { f.cx1 = f.cx0; f.cz1 = f.cz0; } // no core
+    return f;
 }
-void fused_rtm_batch(ouc_ctx *, const ouc_shot *, int, float *, float *, float *) {
-    throw ouc::Error(OUC_UNSUPPORTED, "fused engine not built yet");
+
+// 64 x 4 tiles (origin i0, j0) over the update region holding at least one cell
+// of `want`.
+template <class Pred>
+std::vector<int2> tile_list(const FGrid &g, Pred want) {
+    std::vector<int2> out;
+    for (int i0 = g.N; i0 < g.Xn - g.N; i0 += 4)
+        for (int j0 = g.N; j0 < g.Zn - g.N; j0 += 64) {
+            bool any = false;
+            for (int i = i0; i < std::min(i0 + 4, g.Xn - g.N) && !any; ++i)
+                for (int j = j0; j < std::min(j0 + 64, g.Zn - g.N) && !any; ++j) any = want(i, j);
+            if (any) out.push_back(make_int2(i0, j0));
+        }
+    return out;
 }
-double fused_own_bytes(const ouc_ctx *) { return 0; }
+
+size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
+    const Grid &g = c->g;
+    const size_t plane = g.pitched() * sizeof(float);
+    size_t b = 2 * kWaveArrays * plane + 6 * plane + 3 * plane;
+    const size_t cells = (size_t)c->ring.ncells;
+    b += (size_t)g.nt * RS_NF * cells * (fp16 ? sizeof(__half) : sizeof(float));
+    if (fp16) b += (size_t)g.nt * RS_NF * ((cells + 63) / 64);
+    b += (size_t)2 * g.nt * g.nx * sizeof(float) + (size_t)g.nt * sizeof(float) + g.nx * sizeof(int32_t);
+    return b;
+}
+
+void ensure(ouc_ctx *c, int shots) {
+    FusedState *st = fs(c);
+    const int want16 = c->p.strip_precision == OUC_STRIP_FP16;
+    if (st->cap >= shots && st->fp16 == want16) return;
+    const Grid &g = c->g;
+    st->g = make_fgrid(c);
+    st->plane = st->g.plane;
+    st->cap = shots;
+    st->fp16 = want16;
+    wave_layout(st->mem_s, st->S, st->plane, shots);
+    wave_layout(st->mem_r, st->R, st->plane, shots);
+    const size_t span = (size_t)st->plane * shots * sizeof(float);
+    for (auto &lv : st->vp)
+        for (auto &b : lv) b.alloc(span);
+    for (auto &b : st->acc) b.alloc(span);
+    st->nslots = g.nt;
+    st->nchunks = (c->ring.ncells + 63) / 64;
+    st->shot_strip = (long)st->nslots * RS_NF * c->ring.ncells;
+    st->shot_chunks = (long)st->nslots * RS_NF * st->nchunks;
+    st->strips32.release(); st->strips16.release(); st->e8.release();
+    if (want16) {
+        st->strips16.alloc((size_t)st->shot_strip * shots * sizeof(__half));
+        st->e8.alloc((size_t)st->shot_chunks * shots);
+    } else {
+        st->strips32.alloc((size_t)st->shot_strip * shots * sizeof(float));
+    }
+    st->rec.alloc((size_t)shots * 2 * g.nt * g.nx * sizeof(float));
+    st->srcpos.alloc((size_t)shots * 2 * sizeof(int));
+    st->wav.alloc((size_t)shots * g.nt * sizeof(float));
+    st->counts.alloc((size_t)shots * g.nx * sizeof(int32_t));
+    st->img.alloc((size_t)4 * g.nx * g.nz * sizeof(float));
+    const FGrid &f = st->g;
+    auto core = [&](int i, int j) { return i >= f.cx0 && i < f.cx1 && j >= f.cz0 && j < f.cz1; };
+    auto deep = [&](int i, int j) {
+        return i >= f.cx0 + g.N && i < f.cx1 - g.N && j >= f.cz0 + g.N && j < f.cz1 - g.N;
+    };
+    std::vector<int2> ts = tile_list(f, [&](int i, int j) { return !deep(i, j); });
+    std::vector<int2> tv = tile_list(f, [&](int i, int j) { return !core(i, j); });
+    st->ntiles_s = (int)ts.size();
+    st->ntiles_v = (int)tv.size();
+    st->tiles_s.alloc(ts.size() * sizeof(int2));
+    st->tiles_v.alloc(tv.size() * sizeof(int2));
+    OUC_HIP(hipMemcpy(st->tiles_s.ptr, ts.data(), ts.size() * sizeof(int2), hipMemcpyHostToDevice));
+    OUC_HIP(hipMemcpy(st->tiles_v.ptr, tv.data(), tv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    // reconstruction band: stress on the ring frame minus [L+3N, Xn-L-3N)^2,
+    // velocity on the interior minus the recon core [L+2N, Xn-L-2N)^2
+    const int L = g.L, N = g.N;
+    std::vector<int2> trs, trv;
+    for (int i0 = L - N; i0 < g.Xn - L + N; i0 += 4)
+        for (int j0 = L - N; j0 < g.Zn - L + N; j0 += 64) {
+            bool s_any = false, v_any = false;
+            for (int i = i0; i < std::min(i0 + 4, g.Xn - L + N); ++i)
+                for (int j = j0; j < std::min(j0 + 64, g.Zn - L + N); ++j) {
+                    bool deep3 = i >= L + 3 * N && i < g.Xn - L - 3 * N && j >= L + 3 * N && j < g.Zn - L - 3 * N;
+                    bool interior = i >= L && i < g.Xn - L && j >= L && j < g.Zn - L;
+                    bool core2 = i >= L + 2 * N && i < g.Xn - L - 2 * N && j >= L + 2 * N && j < g.Zn - L - 2 * N;
+                    s_any |= !deep3;
+                    v_any |= interior && !core2;
+                }
+            if (s_any) trs.push_back(make_int2(i0, j0));
+            if (v_any) trv.push_back(make_int2(i0, j0));
+        }
+    st->ntiles_rs = (int)trs.size();
+    st->ntiles_rv = (int)trv.size();
+    st->tiles_rs.alloc(trs.size() * sizeof(int2));
+    st->tiles_rv.alloc(trv.size() * sizeof(int2));
+    OUC_HIP(hipMemcpy(st->tiles_rs.ptr, trs.data(), trs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    OUC_HIP(hipMemcpy(st->tiles_rv.ptr, trv.data(), trv.size() * sizeof(int2), hipMemcpyHostToDevice));
+}
+
+KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
+    KStrips s{};
+    s.f32 = st->strips32.f();
+    s.f16 = st->strips16.as<__half>();
+    s.e8 = st->e8.as<int8_t>();
+    s.ncells = c->ring.ncells;
+    s.nchunks = st->nchunks;
+    s.fp16 = st->fp16;
+    return s;
+}
+
+// Direct-wave mute for a batch: counts[shot][nx].
+__global__ void k_fmute(int nx, int nt, const int32_t *__restrict__ counts, float *__restrict__ rec) {
+    int ix = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y, s = blockIdx.z;
+    if (ix >= nx || t >= nt) return;
+    if (t < counts[(long)s * nx + ix]) {
+        long o = ((long)s * 2 * nt + t) * nx + ix;
+        rec[o] = 0.f;
+        rec[o + (long)nt * nx] = 0.f;
+    }
+}
+
+struct Launch {
+    dim3 core, band_s, band_v, recon_s, recon_v, recon_core, full;
+    bool has_core, has_recon_core;
+    FGrid rg; // the grid with the reconstruction core bounds
+};
+
+Launch launches(const FusedState *st, int B) {
+    const FGrid &f = st->g;
+    Launch L{};
+    const int TZ = 256 - 4 * f.N;
+    L.has_core = f.cx1 > f.cx0 && f.cz1 > f.cz0;
+    L.core = dim3((f.cx1 - f.cx0 + kCoreSX - 1) / kCoreSX, (f.cz1 - f.cz0 + TZ - 1) / TZ, B);
+    L.band_s = dim3(st->ntiles_s, 1, B);
+    L.band_v = dim3(st->ntiles_v, 1, B);
+    L.recon_s = dim3(st->ntiles_rs, 1, B);
+    L.recon_v = dim3(st->ntiles_rv, 1, B);
+    L.rg = f;
+    L.rg.cx0 = f.L + 2 * f.N; L.rg.cx1 = f.Xn - f.L - 2 * f.N;
+    L.rg.cz0 = f.L + 2 * f.N; L.rg.cz1 = f.Zn - f.L - 2 * f.N;
+    L.has_recon_core = L.rg.cx1 > L.rg.cx0 && L.rg.cz1 > L.rg.cz0;
+    if (!L.has_recon_core) { L.rg.cx1 = L.rg.cx0; L.rg.cz1 = L.rg.cz0; }
+    L.recon_core = dim3((L.rg.cx1 - L.rg.cx0 + kCoreSX - 1) / kCoreSX, (L.rg.cz1 - L.rg.cz0 + TZ - 1) / TZ, B);
+    L.full = dim3((f.Zn + 63) / 64, (f.Xn + 3) / 4, B);
+    return L;
+}
+
+// Sampled timing of k_core in profiling mode: event pairs around every
+// `stride`-th launch.
+struct CoreProbe {
+    std::vector<hipEvent_t> ev;
+    int stride = 1, used = 0;
+    void init(int launches, bool on) {
+        if (!on) return;
+        int slots = std::min(launches, 512);
+        stride = std::max(1, launches / slots);
+        ev.resize(2 * slots);
+        for (auto &e : ev) OUC_HIP(hipEventCreate(&e));
+    }
+    bool want(int n) const { return !ev.empty() && n % stride == 0 && used < (int)ev.size() / 2; }
+    double total_ms() {
+        double t = 0;
+        for (int a = 0; a < used; ++a) {
+            float ms = 0;
+            OUC_HIP(hipEventElapsedTime(&ms, ev[2 * a], ev[2 * a + 1]));
+            t += ms;
+        }
+        return t;
+    }
+    ~CoreProbe() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, const FStep &a, CoreProbe &pr,
+                      int &nlaunch) {
+    const FGrid &f = st->g;
+    const KModel km = c->kmodel();
+    hipStream_t s = c->stream;
+    OUC_FDISPATCH(f.N, k_band_stress<NN><<<L.band_s, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_s.as<int2>(), a));
+    if (L.has_core) {
+        bool p = pr.want(nlaunch++);
+        if (p) OUC_HIP(hipEventRecord(pr.ev[2 * pr.used], s));
+        OUC_FDISPATCH(f.N, k_core<NN, 0><<<L.core, 256, 0, s>>>(f, w, c->p2.f(), c->q2.f(), a, FRecon{}));
+        if (p) { OUC_HIP(hipEventRecord(pr.ev[2 * pr.used + 1], s)); ++pr.used; }
+    }
+    OUC_FDISPATCH(f.N, k_band_velocity<NN><<<L.band_v, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_v.as<int2>(), a));
+}
+
+// Runs B shots through forward (+ strips) and, with rtm, the backward loop;
+// leaves the accumulators of every shot in st->acc.
+void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe) {
+    FusedState *st = fs(c);
+    const Grid &g = c->g;
+    const FGrid &f = st->g;
+    hipStream_t s = c->stream;
+    const Launch L = launches(st, B);
+    const bool exact = c->p.semantics == OUC_SEM_EXACT;
+    const int nt = g.nt;
+    // per-shot inputs
+    std::vector<int> pos(2 * B);
+    std::vector<float> wavs((size_t)B * nt);
+    std::vector<int32_t> counts((size_t)B * g.nx, 0);
+    int zr = -1;
+    for (int b = 0; b < B; ++b) {
+        const ouc_shot &sh = shots[b];
+        pos[2 * b] = sh.sx + g.L;
+        pos[2 * b + 1] = sh.sz + g.L;
+        std::vector<float> w;
+        if (sh.wavelet) w.assign(sh.wavelet, sh.wavelet + nt);
+        else ricker(nt, sh.fm, g.dt, w);
+        std::copy(w.begin(), w.end(), wavs.begin() + (size_t)b * nt);
+        if (zr < 0) zr = sh.rec_z + g.L;
+        if (sh.rec_z + g.L != zr) throw Error(OUC_INVALID_ARG, "all shots of a batch must share the receiver row");
+        if (sh.mute) {
+            std::vector<int32_t> cnt;
+            mute_counts(g, sh, c->vp_pad[(size_t)(sh.sx + g.L) * g.P + zr], cnt);
+            std::copy(cnt.begin(), cnt.end(), counts.begin() + (size_t)b * g.nx);
+        }
+    }
+    OUC_HIP(hipMemcpyAsync(st->srcpos.ptr, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    OUC_HIP(hipMemcpyAsync(st->wav.ptr, wavs.data(), wavs.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    OUC_HIP(hipMemcpyAsync(st->counts.ptr, counts.data(), counts.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    const size_t span = (size_t)st->plane * B * sizeof(float);
+    OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
+    const KRing kr = c->kring();
+    const KStrips ks0 = base_strips(st, c);
+    const int rblocks = (int)((c->ring.ncells + 255) / 256);
+    int nlaunch = 0;
+
+    // forward
+    FStep a{};
+    a.src = st->srcpos.as<int>();
+    a.wav = st->wav.f();
+    a.zr = zr;
+    a.rec = st->rec.f();
+    a.ncomp = ncomp;
+    int rb = 0;
+    for (int t = 0; t < nt; ++t) {
+        a.t = t;
+        a.rb = rb;
+        wave_step_probed(c, st, L, st->S, a, probe, nlaunch);
+        if (rtm) {
+            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks, t, 0,
+                                                               0);
+            if (t < nt - 1)
+                k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks,
+                                                                   t, 1 - rb, 1);
+        }
+        rb = 1 - rb;
+    }
+    OUC_HIP(hipGetLastError());
+    if (!rtm) return;
+    c->last.forward_ms = 0; // filled by the caller from events
+    OUC_HIP(hipEventRecord(c->ev[1], s));
+    {
+        dim3 gm((g.nx + 255) / 256, nt, B);
+        k_fmute<<<gm, 256, 0, s>>>(g.nx, nt, st->counts.as<int32_t>(), st->rec.f());
+    }
+    // hand-off: V^nt in buffer rb; V^{nt-1} into 1-rb; source P levels nt, nt-1
+    k_funstep<<<L.full, dim3(64, 4), 0, s>>>(f, st->S, rb, st->vp[nt % 3][0].f(), st->vp[nt % 3][1].f(),
+                                             st->vp[(nt - 1) % 3][0].f(), st->vp[(nt - 1) % 3][1].f());
+    int rbs = 1 - rb;
+    OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
+    for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, span, s));
+    FStep r{};
+    r.zr = zr;
+    r.inj = st->rec.f();
+    r.npp = st->acc[0].f(); r.nps = st->acc[1].f(); r.den = st->acc[2].f();
+    int rbr = 0;
+    for (int t = nt - 1; t >= 0; --t) {
+        if (t < nt - 1) {
+            // theta^{t+1} from V^{t+1}: ring totals from slot t+1 (FAITHFUL, one step late as the
+            // reference) or slot t (EXACT); the hand-off step uses the forward's own state
+            FRecon rc{};
+            rc.ring = kr;
+            rc.strips = ks0;
+            rc.shot_strip = st->shot_strip;
+            rc.shot_chunks = st->shot_chunks;
+            const bool handoff = t == nt - 2;
+            rc.slot_v = handoff ? -1 : (exact ? t : t + 1);
+            rc.slot_th = t + 1;
+            rc.exact = (exact || handoff) ? 1 : 0;
+            rc.vpx = st->vp[t % 3][0].f();
+            rc.vpz = st->vp[t % 3][1].f();
+            FStep ra{};
+            ra.t = t + 1;
+            ra.rb = rbs;
+            ra.src = st->srcpos.as<int>();
+            ra.wav = st->wav.f();
+            OUC_FDISPATCH(f.N, k_recon_stress_f<NN><<<L.recon_s, dim3(64, 4), 0, s>>>(
+                                   f, st->S, c->p2.f(), c->q2.f(), rc, ra, st->tiles_rs.as<int2>()));
+            if (L.has_recon_core)
+                OUC_FDISPATCH(f.N, k_core<NN, 1><<<L.recon_core, 256, 0, s>>>(L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
+            OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, s>>>(f, st->S, c->q2.f(), rc, ra,
+                                                                                        st->tiles_rv.as<int2>()));
+            rbs = 1 - rbs;
+        }
+        const int lev = exact ? t + 1 : std::min(t + 2, nt);
+        r.t = t;
+        r.rb = rbr;
+        r.spx = st->vp[lev % 3][0].f();
+        r.spz = st->vp[lev % 3][1].f();
+        CoreProbe none;
+        int nl2 = 0;
+        wave_step_probed(c, st, L, st->R, r, none, nl2);
+        rbr = 1 - rbr;
+    }
+    OUC_HIP(hipGetLastError());
+}
+
+// Images of shot b of the batch into c->img[0..3].
+void shot_images(ouc_ctx *c, FusedState *st, int b) {
+    const Grid &g = c->g;
+    const KGrid kg = c->kgrid();
+    const size_t off = (size_t)b * st->plane;
+    dim3 grid((g.nz + 255) / 256, g.nx);
+    k_image<<<grid, 256, 0, c->stream>>>(kg, st->acc[0].f() + off, st->acc[1].f() + off, st->acc[2].f() + off,
+                                         c->img[0].f(), c->img[1].f());
+    if (c->p.laplacian == OUC_LAP_STENCIL) {
+        k_laplace5<<<grid, 256, 0, c->stream>>>(g.nx, g.nz, g.dx, c->img[0].f(), c->img[2].f());
+        k_laplace5<<<grid, 256, 0, c->stream>>>(g.nx, g.nz, g.dx, c->img[1].f(), c->img[3].f());
+    } else {
+        lap_fft(c, c->img[0].f(), c->img[2].f());
+        lap_fft(c, c->img[1].f(), c->img[3].f());
+    }
+    OUC_HIP(hipGetLastError());
+}
+
+int default_capacity(ouc_ctx *c, int nshots) {
+    size_t freeb = 0, total = 0;
+    OUC_HIP(hipMemGetInfo(&freeb, &total));
+    // keep what this context already holds available for reuse
+    FusedState *st = fs(c);
+    size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes;
+    size_t avail = (size_t)((freeb + held) * 0.85);
+    size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
+    int cap = (int)std::max<size_t>(1, avail / per);
+    return std::min(cap, nshots);
+}
+
+} // namespace
+
+void fused_init(ouc_ctx *c) { c->fused = new FusedState(); }
+
+void fused_destroy(ouc_ctx *c) {
+    delete fs(c);
+    c->fused = nullptr;
+}
+
+double fused_own_bytes(const ouc_ctx *c) {
+    const Grid &g = c->g;
+    const double upd = (double)(g.Xn - 2 * g.N) * (g.Zn - 2 * g.N), interior = (double)g.nx * g.nz;
+    const double band = upd - (double)std::max(0, g.Xn - 2 * (g.L + g.N)) * std::max(0, g.Zn - 2 * (g.L + g.N));
+    const double core = upd - band;
+    // core: state 8 R + 8 W, vp^2, vs^2; band: + 6 stress W/R, F/O, damping;
+    // receiver core: + source P level and accumulators; reconstruction: stress
+    // store then reverse step
+    const double fwd = core * 72 + band * (72 + 4 * (6 + 6 + 2 + 8 + 16));
+    const double rcv = fwd + interior * (8 + 24);
+    const double rec = interior * 4 * (2 + 2 + 6 + 6 + 1 + 8 + 8 + 2);
+    return (double)g.nt * (fwd + rcv + rec);
+}
+
+void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
+                     float *per_shot_ms) {
+    const Grid &g = c->g;
+    for (int k = 0; k < nshots; ++k) {
+        const ouc_shot *sh = &shots[k];
+        if (sh->sx < 0 || sh->sx >= g.nx || sh->sz < 0 || sh->sz >= g.nz || sh->rec_z < 0 || sh->rec_z >= g.nz)
+            throw Error(OUC_INVALID_ARG, "shot " + std::to_string(k) + " outside the interior");
+        if (!sh->wavelet && !(sh->fm > 0)) throw Error(OUC_INVALID_ARG, "fm must be positive");
+    }
+    const int cap = default_capacity(c, nshots);
+    ensure(c, cap);
+    FusedState *st = fs(c);
+    const size_t ni = (size_t)g.nx * g.nz;
+    DevBuf stack;
+    stack.alloc(2 * ni * sizeof(float));
+    OUC_HIP(hipMemsetAsync(stack.ptr, 0, 2 * ni * sizeof(float), c->stream));
+    c->last = ouc_timings{};
+    CoreProbe probe;
+    probe.init(g.nt * ((nshots + cap - 1) / cap), c->profiling);
+    OUC_HIP(hipEventRecord(c->ev[0], c->stream));
+    double fwd_ms = 0, bwd_ms = 0, img_ms = 0;
+    for (int b0 = 0; b0 < nshots; b0 += cap) {
+        const int B = std::min(cap, nshots - b0);
+        OUC_HIP(hipEventRecord(c->ev[4], c->stream));
+        run_batch(c, shots + b0, B, true, 2, probe);
+        OUC_HIP(hipEventRecord(c->ev[2], c->stream));
+        for (int b = 0; b < B; ++b) {
+            shot_images(c, st, b);
+            k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[2].f(), stack.f());
+            k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[3].f(), stack.f() + ni);
+        }
+        OUC_HIP(hipEventRecord(c->ev[3], c->stream));
+        OUC_HIP(hipStreamSynchronize(c->stream));
+        float a = 0, b = 0, d = 0, e = 0;
+        OUC_HIP(hipEventElapsedTime(&a, c->ev[4], c->ev[1]));
+        OUC_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        OUC_HIP(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+        OUC_HIP(hipEventElapsedTime(&e, c->ev[0], c->ev[3]));
+        fwd_ms += a; bwd_ms += b; img_ms += d;
+        if (per_shot_ms)
+            for (int k = 0; k < B; ++k) per_shot_ms[b0 + k] = e;
+    }
+    OUC_HIP(hipEventRecord(c->ev[5], c->stream));
+    if (stack_pp) OUC_HIP(hipMemcpyAsync(stack_pp, stack.ptr, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (stack_ps)
+        OUC_HIP(hipMemcpyAsync(stack_ps, stack.f() + ni, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+    float tot = 0;
+    OUC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+    c->last.total_ms = tot;
+    c->last.forward_ms = fwd_ms;
+    c->last.backward_ms = bwd_ms;
+    c->last.image_ms = img_ms;
+    c->last.step_launches = (int64_t)2 * g.nt * ((nshots + cap - 1) / cap);
+    c->last.step_kernel_ms = probe.total_ms();
+    c->last.step_kernel_launches = probe.used;
+    const double padded = (double)g.Xn * g.Zn, interior = (double)g.nx * g.nz, cpml = padded - interior;
+    c->last.cell_updates = (double)nshots * g.nt * (2 * padded + interior);
+    c->last.canonical_bytes = (double)nshots * g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
+    c->last.own_bytes = nshots * fused_own_bytes(c);
+}
+
+// Single shot through the fused path (a batch of one), returning its images.
+void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, float *img_pp, float *img_ps, float *lap_pp, float *lap_ps) {
+    ensure(c, std::max(1, fs(c)->cap));
+    FusedState *st = fs(c);
+    CoreProbe probe;
+    probe.init(c->g.nt, c->profiling);
+    c->last = ouc_timings{};
+    OUC_HIP(hipEventRecord(c->ev[0], c->stream));
+    run_batch(c, s, 1, true, 2, probe);
+    OUC_HIP(hipEventRecord(c->ev[2], c->stream));
+    shot_images(c, st, 0);
+    OUC_HIP(hipEventRecord(c->ev[3], c->stream));
+    const size_t ni = (size_t)c->g.nx * c->g.nz;
+    float *outs[4] = {img_pp, img_ps, lap_pp, lap_ps};
+    for (int a = 0; a < 4; ++a)
+        if (outs[a]) OUC_HIP(hipMemcpyAsync(outs[a], c->img[a].ptr, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+    float fw = 0, bw = 0, im = 0, tot = 0;
+    OUC_HIP(hipEventElapsedTime(&fw, c->ev[0], c->ev[1]));
+    OUC_HIP(hipEventElapsedTime(&bw, c->ev[1], c->ev[2]));
+    OUC_HIP(hipEventElapsedTime(&im, c->ev[2], c->ev[3]));
+    OUC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[3]));
+    c->last.forward_ms = fw; c->last.backward_ms = bw; c->last.image_ms = im; c->last.total_ms = tot;
+    c->last.step_launches = 2 * (int64_t)c->g.nt;
+    c->last.step_kernel_ms = probe.total_ms();
+    c->last.step_kernel_launches = probe.used;
+    const Grid &g = c->g;
+    const double padded = (double)g.Xn * g.Zn, interior = (double)g.nx * g.nz, cpml = padded - interior;
+    c->last.cell_updates = (double)g.nt * (2 * padded + interior);
+    c->last.canonical_bytes = (double)g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
+    c->last.own_bytes = fused_own_bytes(c);
+}

@@ -1,0 +1,640 @@
+// Fused-engine HIP kernels.
+//
+// Arithmetic form: the state of a wavefield is the total velocity V = Vp + Vs
+// and the S part Vs, each stepped as (value, increment) with
+// V'' = (Tx - OTx, Tz - OTz) + (Wz - OWz, -Wx + OWx) + 2K and
+// Vs'' = (Wz - OWz, -Wx + OWx) + K, where K = (C - E - OC + OE, B - A - OB + OA)
+// of the reference (P-S-seperating.cu:291-294); Vp = V - Vs.  K's non-psi part
+// is evaluated in the product-rule form D((vs^2 - vs^2_0) g), equal in exact
+// arithmetic because the x and z staggered operators commute.
+//
+// Per step and wavefield there are three launches:
+//   k_band_stress   stress on the CPML band (every cell that is not "core") plus
+//                   an N-cell margin into the core, stored to global, with the
+//                   CPML F update (forward_s, P-S-seperating.cu:316-384);
+//   k_core          the core [L+N, Xn-L-N) x [L+N, Zn-L-N): stress and velocity
+//                   in one pass, nothing but the state read and written;
+//   k_band_velocity velocity with CPML O terms on the band (forward_u,
+//                   P-S-seperating.cu:205-314).
+// The core's stress halo lies in [L, Xn-L)^2, where the damping is zero, so the
+// core kernel carries no CPML terms.  V is read through stencils by neighbouring
+// blocks, so it is double-buffered (read rb, write 1-rb); every other array is
+// read only at its own cell and is updated in place.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "kernels_plain.hpp"
+
+namespace ouc {
+
+// Geometry shared by the fused kernels.
+struct FGrid {
+    int Xn, Zn, P, L, N, nx, nz, nt;
+    long plane;     // floats per pitched array of one shot
+    float idx, idz, dt, dt2;
+    float c[8];
+    int cx0, cx1, cz0, cz1; // core
+};
+
+// One wavefield for all shots of a batch; arrays are [shot][plane].
+struct FWave {
+    float *V[2][2];          // [buffer][x, z]
+    float *DV[2], *S[2], *DS[2];
+    float *th, *om, *gxx, *gxz, *gzx, *gzz; // band stress (stored)
+    float *F[4], *O[8];
+};
+
+// Per-step, per-batch arguments.
+struct FStep {
+    int t;                   // time index of this step
+    int rb;                  // V buffer read this step (written: 1 - rb)
+    const int *src;          // [shot][2] padded source cell, or NULL
+    const float *wav;        // [shot][nt] wavelet, or NULL
+    int zr;                  // padded receiver row
+    float *rec;              // FWD: [shot][ncomp][nt][nx] records, or NULL
+    int ncomp;
+    const float *inj;        // RCV: [shot][2][nt][nx] records to inject, or NULL
+    const float *spx, *spz;  // RCV: source P velocity level [shot][plane], or NULL
+    float *npp, *nps, *den;  // RCV: imaging accumulators [shot][plane]
+};
+
+__device__ __forceinline__ bool in_update(const FGrid &g, int i, int j) {
+    return i >= g.N && i < g.Xn - g.N && j >= g.N && j < g.Zn - g.N;
+}
+
+// V as seen by the stress pass at (i, j): the state, zero outside the update
+// region, the injected record on the receiver row (load_record,
+// P-S-seperating.cu:1471-1483).
+__device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const FStep &a, long so, int i, int j,
+                                         float &vx, float &vz) {
+    vx = 0.f;
+    vz = 0.f;
+    if (!in_update(g, i, j)) return;
+    long k = so + (long)i * g.P + j;
+    vx = w.V[a.rb][0][k];
+    vz = w.V[a.rb][1][k];
+    if (a.inj && j == a.zr && i >= g.L && i < g.Xn - g.L) {
+        long o = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
+        vx = a.inj[o];
+        vz = a.inj[o + (long)g.nt * g.nx];
+    }
+}
+
+__device__ __forceinline__ float src_term(const FGrid &g, const FStep &a, int i, int j) {
+    if (!a.src) return 0.f;
+    const int s = blockIdx.z;
+    return (i == a.src[2 * s] && j == a.src[2 * s + 1]) ? a.wav[(long)s * g.nt + a.t] : 0.f;
+}
+
+// Record totals and P/S parts after the update (P-S-seperating.cu:307-311).
+__device__ __forceinline__ void record(const FGrid &g, const FStep &a, int i, int j, float vx, float vz, float sx,
+                                       float sz) {
+    if (!a.rec || j != a.zr || i < g.L || i >= g.Xn - g.L) return;
+    const long plane = (long)g.nt * g.nx;
+    long o = ((long)blockIdx.z * a.ncomp) * plane + (long)a.t * g.nx + (i - g.L);
+    a.rec[o] = vx;
+    a.rec[o + plane] = vz;
+    if (a.ncomp == 6) {
+        a.rec[o + 2 * plane] = vx - sx;
+        a.rec[o + 3 * plane] = vz - sz;
+        a.rec[o + 4 * plane] = sx;
+        a.rec[o + 5 * plane] = sz;
+    }
+}
+
+// corr_v (P-S-seperating.cu:1426-1439) for an interior cell.
+__device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k, int i, int j, float vx, float vz,
+                                          float sx, float sz) {
+    if (!a.spx || i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L) return;
+    float sa = a.spx[k], sb = a.spz[k];
+    float rpx = vx - sx, rpz = vz - sz;
+    a.npp[k] += sa * rpx + sb * rpz;
+    a.den[k] += sa * sa + sb * sb;
+    a.nps[k] += sa * sx + sb * sz;
+}
+
+// ---------------------------------------------------------------------------
+// Core kernel.  Block = 256 lanes along z covering TZ = 256 - 4N output rows
+// plus a 2N halo each side; the block streams along x over SX columns.  At
+// stream position c it loads V at column c, forms the stress at column c - N
+// (x derivatives from a register queue of V, z derivatives through LDS), and
+// updates column c - 2N (x derivatives of the stress from register queues, z
+// derivatives computed at stress time and delayed N columns).
+// ---------------------------------------------------------------------------
+struct FRecon {
+    KRing ring;
+    KStrips strips;      // base for shot 0; per-shot offset below
+    long shot_strip;     // elements between shots' strip stores
+    long shot_chunks;    // exponent entries between shots
+    int slot_v;          // slot of the ring totals (< 0: the state's own, no override)
+    int slot_th;         // slot of the ring stress-like fields
+    int exact;           // add the source term (EXACT semantics, or the hand-off step)
+    float *vpx, *vpz;    // [shot][plane] output level
+};
+
+__device__ __forceinline__ long ring_index(const KRing &r, int i, int j) {
+    const int L = r.L, N = r.N;
+    if (i < L - N || i >= r.Xn - L + N || j < L - N || j >= r.Zn - L + N) return -1;
+    if (i >= L + N && i < r.Xn - L - N && j >= L + N && j < r.Zn - L - N) return -1;
+    if (j < L + N) return (long)(i - (L - N)) * (2 * N) + (j - (L - N));
+    if (j >= r.Zn - L - N) return r.ntop + (long)(i - (L - N)) * (2 * N) + (j - (r.Zn - L - N));
+    const int w = r.nz - 2 * N;
+    if (i < L + N) return 2 * r.ntop + (long)(i - (L - N)) * w + (j - (L + N));
+    return 2 * r.ntop + r.nside + (long)(i - (r.Xn - L - N)) * w + (j - (L + N));
+}
+
+__device__ __forceinline__ KStrips shot_strips(const FRecon &rc) {
+    KStrips s = rc.strips;
+    const long so = (long)blockIdx.z * rc.shot_strip, eo = (long)blockIdx.z * rc.shot_chunks;
+    if (s.f32) s.f32 += so;
+    if (s.f16) s.f16 += so;
+    if (s.e8) s.e8 += eo;
+    return s;
+}
+
+// Totals seen by the reconstruction's stress pass at (i, j).
+__device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s, int rb,
+                                          long so, int i, int j, float &vx, float &vz) {
+    if (!in_update(g, i, j)) { vx = vz = 0.f; return; }
+    long r = rc.slot_v >= 0 ? ring_index(rc.ring, i, j) : -1;
+    if (r >= 0) {
+        vx = strip_get(s, rc.slot_v, RS_VX, r);
+        vz = strip_get(s, rc.slot_v, RS_VZ, r);
+        return;
+    }
+    long k = so + (long)i * g.P + j;
+    vx = w.V[rb][0][k];
+    vz = w.V[rb][1][k];
+}
+
+constexpr int kCoreSX = 64;
+constexpr int kCorePF = 2; // columns of loads in flight ahead of use
+
+// RECON = 1 runs the source reconstruction (reshot_u + reshot_s,
+// P-S-seperating.cu:890-1042) through the same pipeline: ring totals come from
+// the strips, the step is the reverse one, and the new source P velocity level
+// is written for the imaging condition.  Its core is [L+2N, Xn-L-2N)^2, so every
+// stress cell it forms lies off the ring.
+template <int N, int RECON>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float *__restrict__ p2,
+                                              const float *__restrict__ q2, FStep a, FRecon rc) {
+    constexpr int TZ = 256 - 4 * N;
+    constexpr int Q = 2 * N + 1;
+    constexpr int PF = kCorePF;
+    __shared__ float lv[2][256];
+    __shared__ float ls[5][256];
+    const int l = threadIdx.x;
+    const int z = g.cz0 + blockIdx.y * TZ - 2 * N + l;
+    const int xs = g.cx0 + blockIdx.x * kCoreSX;
+    const int xe = min(xs + kCoreSX, g.cx1);
+    const long so = (long)blockIdx.z * g.plane;
+    const float *c = g.c;
+    const int wb = 1 - a.rb;
+    const bool z_out = z >= g.cz0 && z < g.cz1 && l >= 2 * N && l < 256 - 2 * N;
+    const bool z_st = l >= N && l < 256 - N && z >= N && z < g.Zn - N;
+    KStrips rs{};
+    if (RECON) rs = shot_strips(rc);
+    auto vin = [&](int i, float &vx, float &vz) {
+        if (RECON) recon_vin(g, w, rc, rs, a.rb, so, i, z, vx, vz);
+        else load_vin(g, w, a, so, i, z, vx, vz);
+    };
+
+    float qvx[Q], qvz[Q];                         // V at columns cc-2N..cc
+    float xth[Q], xom[Q], xgzz[Q], xgxz[Q], xq[Q]; // stress at columns cs-2N..cs
+    float zq[N + 1][4];                           // z-terms Tz, Wz, KXz, KZz, delayed N
+    float pv[PF][2], ppq[PF][2], pst[PF][6];      // prefetched V, (p, q), state
+#pragma unroll
+    for (int m = 0; m < Q; ++m) { qvx[m] = qvz[m] = 0.f; xth[m] = xom[m] = xgzz[m] = xgxz[m] = xq[m] = 0.f; }
+#pragma unroll
+    for (int m = 0; m <= N; ++m) zq[m][0] = zq[m][1] = zq[m][2] = zq[m][3] = 0.f;
+    auto load_pq = [&](int cs, float *o) {
+        o[0] = o[1] = 0.f;
+        if (z_st && cs >= xs - N && cs < xe + N) {
+            o[0] = p2[(long)cs * g.P + z];
+            o[1] = q2[(long)cs * g.P + z];
+        }
+    };
+    auto load_state = [&](int cu, float *o) {
+        if (z_out && cu >= xs && cu < xe) {
+            const long k = so + (long)cu * g.P + z;
+            o[0] = w.DV[0][k]; o[1] = w.DV[1][k]; o[2] = w.S[0][k]; o[3] = w.S[1][k];
+            o[4] = w.DS[0][k]; o[5] = w.DS[1][k];
+        } else {
+            o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0.f;
+        }
+    };
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+        const int cc = xs - 2 * N + f;
+        vin(cc, pv[f][0], pv[f][1]);
+        load_pq(cc - N, ppq[f]);
+        load_state(cc - 2 * N, pst[f]);
+    }
+
+    for (int cc = xs - 2 * N; cc < xe + 2 * N; ++cc) {
+        // 1. V at column cc (loaded PF columns ago); issue the loads of column cc + PF
+#pragma unroll
+        for (int m = 0; m < Q - 1; ++m) { qvx[m] = qvx[m + 1]; qvz[m] = qvz[m + 1]; }
+        qvx[Q - 1] = pv[0][0];
+        qvz[Q - 1] = pv[0][1];
+        const float pp = ppq[0][0], qq0 = ppq[0][1];
+        float st6[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) st6[e] = pst[0][e];
+#pragma unroll
+        for (int f = 0; f < PF - 1; ++f) {
+            pv[f][0] = pv[f + 1][0]; pv[f][1] = pv[f + 1][1];
+            ppq[f][0] = ppq[f + 1][0]; ppq[f][1] = ppq[f + 1][1];
+#pragma unroll
+            for (int e = 0; e < 6; ++e) pst[f][e] = pst[f + 1][e];
+        }
+        vin(cc + PF, pv[PF - 1][0], pv[PF - 1][1]);
+        load_pq(cc + PF - N, ppq[PF - 1]);
+        load_state(cc + PF - 2 * N, pst[PF - 1]);
+        // 2. stress at column cs = cc - N
+        const int cs = cc - N;
+        lv[0][l] = qvx[N];
+        lv[1][l] = qvz[N];
+        __syncthreads();
+        float th = 0.f, om = 0.f, gzx = 0.f, gxx = 0.f, qq = 0.f, gzz = 0.f, gxz = 0.f;
+        if (cc >= xs && z_st) {
+            float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
+#pragma unroll
+            for (int m = 1; m <= N; ++m) {
+                s1 += c[m - 1] * (qvx[N + m - 1] - qvx[N - m]);    // D-x Vx
+                s2 += c[m - 1] * (qvz[N + m] - qvz[N - m + 1]);    // D+x Vz
+                s3 += c[m - 1] * (lv[0][l + m] - lv[0][l - m + 1]); // D+z Vx
+                s4 += c[m - 1] * (lv[1][l + m - 1] - lv[1][l - m]); // D-z Vz
+            }
+            gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
+            qq = qq0;
+            th = pp * (gxx + gzz);
+            if (!RECON || rc.exact) th += src_term(g, a, cs, z);
+            om = qq * (gxz - gzx);
+        }
+        ls[0][l] = th; ls[1][l] = om; ls[2][l] = gzx; ls[3][l] = gxx; ls[4][l] = qq;
+        __syncthreads();
+        // z-derivative terms at column cs (q0 = q at the cell)
+        float tz = 0.f, wz = 0.f, kxz = 0.f, kzz = 0.f;
+        if (cc >= xs && l >= 2 * N && l < 256 - 2 * N) {
+            const float q0 = qq;
+#pragma unroll
+            for (int m = 1; m <= N; ++m) {
+                tz += c[m - 1] * (ls[0][l + m] - ls[0][l - m + 1]);
+                wz += c[m - 1] * (ls[1][l + m - 1] - ls[1][l - m]);
+                kxz += c[m - 1] * (ls[2][l + m - 1] * (ls[4][l + m - 1] - q0) - ls[2][l - m] * (ls[4][l - m] - q0));
+                kzz += c[m - 1] * (ls[3][l + m] * (ls[4][l + m] - q0) - ls[3][l - m + 1] * (ls[4][l - m + 1] - q0));
+            }
+            tz *= g.idz; wz *= g.idz; kxz *= g.idz; kzz *= g.idz;
+        }
+#pragma unroll
+        for (int m = 0; m < Q - 1; ++m) {
+            xth[m] = xth[m + 1]; xom[m] = xom[m + 1]; xgzz[m] = xgzz[m + 1]; xgxz[m] = xgxz[m + 1]; xq[m] = xq[m + 1];
+        }
+        xth[Q - 1] = th; xom[Q - 1] = om; xgzz[Q - 1] = gzz; xgxz[Q - 1] = gxz; xq[Q - 1] = qq;
+#pragma unroll
+        for (int m = 0; m < N; ++m) { zq[m][0] = zq[m + 1][0]; zq[m][1] = zq[m + 1][1]; zq[m][2] = zq[m + 1][2]; zq[m][3] = zq[m + 1][3]; }
+        zq[N][0] = tz; zq[N][1] = wz; zq[N][2] = kxz; zq[N][3] = kzz;
+        // 3. update at column cu = cc - 2N
+        const int cu = cc - 2 * N;
+        if (cu >= xs && cu < xe && z_out) {
+            const float q0 = xq[N];
+            float tx = 0.f, wx = 0.f, kxx = 0.f, kzx = 0.f;
+#pragma unroll
+            for (int m = 1; m <= N; ++m) {
+                tx += c[m - 1] * (xth[N + m] - xth[N - m + 1]);
+                wx += c[m - 1] * (xom[N + m - 1] - xom[N - m]);
+                kxx += c[m - 1] * (xgzz[N + m] * (xq[N + m] - q0) - xgzz[N - m + 1] * (xq[N - m + 1] - q0));
+                kzx += c[m - 1] * (xgxz[N + m - 1] * (xq[N + m - 1] - q0) - xgxz[N - m] * (xq[N - m] - q0));
+            }
+            tx *= g.idx; wx *= g.idx; kxx *= g.idx; kzx *= g.idx;
+            const float Kx = zq[0][2] - kxx, Kz = kzx - zq[0][3];
+            const float hx = zq[0][1], hz = -wx;
+            const long k = so + (long)cu * g.P + z;
+            float vx = qvx[0], vz = qvz[0];
+            if (a.inj && z == a.zr) { vx = w.V[a.rb][0][k]; vz = w.V[a.rb][1][k]; } // state, not the injected record
+            const float dt2 = g.dt2;
+            const float sg = RECON ? -1.f : 1.f; // the reverse step runs the increments backwards
+            float d;
+            d = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); w.DV[0][k] = d; vx = vx + sg * d;
+            d = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); w.DV[1][k] = d; vz = vz + sg * d;
+            w.V[wb][0][k] = vx;
+            w.V[wb][1][k] = vz;
+            d = st6[4] + sg * dt2 * (hx + Kx); w.DS[0][k] = d; const float sx = st6[2] + sg * d; w.S[0][k] = sx;
+            d = st6[5] + sg * dt2 * (hz + Kz); w.DS[1][k] = d; const float sz = st6[3] + sg * d; w.S[1][k] = sz;
+            if (RECON) {
+                rc.vpx[k] = vx - sx;
+                rc.vpz[k] = vz - sz;
+            } else {
+                record(g, a, cu, z, vx, vz, sx, sz);
+                correlate(g, a, k, cu, z, vx, vz, sx, sz);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Band kernels: 64 x 4 tiles from a host-built list (tile origin i0, j0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool in_core(const FGrid &g, int i, int j) {
+    return i >= g.cx0 && i < g.cx1 && j >= g.cz0 && j < g.cz1;
+}
+
+// Stress with CPML on band cells and the core's outer N-cell margin.
+template <int N>
+__global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                     FStep a) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (!in_update(g, i, j)) return;
+    // skip core cells farther than N from the band
+    if (i >= g.cx0 + N && i < g.cx1 - N && j >= g.cz0 + N && j < g.cz1 - N) return;
+    const long so = (long)blockIdx.z * g.plane;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    const float *c = g.c;
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
+#pragma unroll
+    for (int mm = 1; mm <= N; ++mm) {
+        float a0, b0, a1, b1;
+        load_vin(g, w, a, so, i + mm - 1, j, a0, b0);
+        load_vin(g, w, a, so, i - mm, j, a1, b1);
+        s1 += c[mm - 1] * (a0 - a1); // D-x Vx
+        load_vin(g, w, a, so, i + mm, j, a0, b0);
+        load_vin(g, w, a, so, i - mm + 1, j, a1, b1);
+        s2 += c[mm - 1] * (b0 - b1); // D+x Vz
+        load_vin(g, w, a, so, i, j + mm, a0, b0);
+        load_vin(g, w, a, so, i, j - mm + 1, a1, b1);
+        s3 += c[mm - 1] * (a0 - a1); // D+z Vx
+        load_vin(g, w, a, so, i, j + mm - 1, a0, b0);
+        load_vin(g, w, a, so, i, j - mm, a1, b1);
+        s4 += c[mm - 1] * (b0 - b1); // D-z Vz
+    }
+    const float gxx = s1 * g.idx, gzx = s2 * g.idx, gxz = s3 * g.idz, gzz = s4 * g.idz;
+    const float a1 = m.ax[k0], a2 = m.az[k0];
+    float Fxx = 0.f, Fzz = 0.f, Fxz = 0.f, Fzx = 0.f;
+    if (a1 != 0.f || a2 != 0.f) {
+        const float dt = g.dt;
+        Fxx = w.F[0][k]; Fzz = w.F[1][k]; Fxz = w.F[2][k]; Fzx = w.F[3][k];
+        Fxx = Fxx + (a1 * gxx - a1 * Fxx) * dt;
+        Fzz = Fzz + (a2 * gzz - a2 * Fzz) * dt;
+        Fxz = Fxz + (a2 * gxz - a2 * Fxz) * dt;
+        Fzx = Fzx + (a1 * gzx - a1 * Fzx) * dt;
+        w.F[0][k] = Fxx; w.F[1][k] = Fzz; w.F[2][k] = Fxz; w.F[3][k] = Fzx;
+    }
+    w.th[k] = m.p[k0] * (gxx + gzz - Fxx - Fzz) + src_term(g, a, i, j);
+    w.om[k] = m.q[k0] * (gxz - gzx - Fxz + Fzx);
+    w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
+}
+
+// Velocity with CPML on band cells.
+template <int N>
+__global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                       FStep a) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (!in_update(g, i, j) || in_core(g, i, j)) return;
+    const long so = (long)blockIdx.z * g.plane;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    const float *c = g.c;
+    const int P = g.P;
+    const float *q = m.q; // per grid, no shot offset
+    // Cg = D-z(q gzx), Eg = D+x(q gzz), Bg = D-x(q gxz), Ag = D+z(q gxx) and the
+    // same of q F (the psi parts of the reference's du fields); KX, KZ are
+    // Cg - Eg and Bg - Ag in the product-rule form (q - q0).
+    float Wx = 0.f, Wz = 0.f, Tx = 0.f, Tz = 0.f, KX = 0.f, KZ = 0.f;
+    float Cg = 0.f, Eg = 0.f, Bg = 0.f, Ag = 0.f, Cf = 0.f, Ef = 0.f, Bf = 0.f, Af = 0.f;
+    const float q0 = q[k0];
+#pragma unroll
+    for (int mm = 1; mm <= N; ++mm) {
+        const float cm = c[mm - 1];
+        const long xp = (long)mm * P, xm1 = (long)(mm - 1) * P;
+        Wx += cm * (w.om[k + xm1] - w.om[k - xp]);
+        Wz += cm * (w.om[k + mm - 1] - w.om[k - mm]);
+        Tx += cm * (w.th[k + xp] - w.th[k - xm1]);
+        Tz += cm * (w.th[k + mm] - w.th[k - mm + 1]);
+        {   // D-z along z: cells k+mm-1, k-mm
+            const float qa = q[k0 + mm - 1], qb = q[k0 - mm];
+            const float ga = w.gzx[k + mm - 1], gb = w.gzx[k - mm];
+            KX += cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idz;
+            Cg += cm * (ga * qa - gb * qb);
+            Cf += cm * (w.F[3][k + mm - 1] * qa - w.F[3][k - mm] * qb);
+        }
+        {   // D+x along x: cells k+mm*P, k-(mm-1)*P
+            const float qa = q[k0 + xp], qb = q[k0 - xm1];
+            const float ga = w.gzz[k + xp], gb = w.gzz[k - xm1];
+            KX -= cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idx;
+            Eg += cm * (ga * qa - gb * qb);
+            Ef += cm * (w.F[1][k + xp] * qa - w.F[1][k - xm1] * qb);
+        }
+        {   // D-x along x: cells k+(mm-1)*P, k-mm*P
+            const float qa = q[k0 + xm1], qb = q[k0 - xp];
+            const float ga = w.gxz[k + xm1], gb = w.gxz[k - xp];
+            KZ += cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idx;
+            Bg += cm * (ga * qa - gb * qb);
+            Bf += cm * (w.F[2][k + xm1] * qa - w.F[2][k - xp] * qb);
+        }
+        {   // D+z along z: cells k+mm, k-mm+1
+            const float qa = q[k0 + mm], qb = q[k0 - mm + 1];
+            const float ga = w.gxx[k + mm], gb = w.gxx[k - mm + 1];
+            KZ -= cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idz;
+            Ag += cm * (ga * qa - gb * qb);
+            Af += cm * (w.F[0][k + mm] * qa - w.F[0][k - mm + 1] * qb);
+        }
+    }
+    Wx *= g.idx; Wz *= g.idz; Tx *= g.idx; Tz *= g.idz;
+    const float Cp = (Cg - Cf) * g.idz, Ep = (Eg - Ef) * g.idx, Bp = (Bg - Bf) * g.idx, Ap = (Ag - Af) * g.idz;
+    const float FX = Cf * g.idz - Ef * g.idx, FZ = Bf * g.idx - Af * g.idz;
+    const float a1 = m.ax[k0], a2 = m.az[k0];
+    float OTx = 0.f, OTz = 0.f, OWz = 0.f, OWx = 0.f, OC = 0.f, OE = 0.f, OB = 0.f, OA = 0.f;
+    if (a1 != 0.f || a2 != 0.f) {
+        const float dt = g.dt;
+        OC = w.O[0][k]; OE = w.O[1][k]; OB = w.O[2][k]; OA = w.O[3][k];
+        OTx = w.O[4][k]; OTz = w.O[5][k]; OWz = w.O[6][k]; OWx = w.O[7][k];
+        OC = OC + (a2 * Cp - a2 * OC) * dt;
+        OE = OE + (a1 * Ep - a1 * OE) * dt;
+        OB = OB + (a1 * Bp - a1 * OB) * dt;
+        OA = OA + (a2 * Ap - a2 * OA) * dt;
+        OTx = OTx + (a1 * Tx - a1 * OTx) * dt;
+        OTz = OTz + (a2 * Tz - a2 * OTz) * dt;
+        OWz = OWz + (a2 * Wz - a2 * OWz) * dt;
+        OWx = OWx + (a1 * Wx - a1 * OWx) * dt;
+        w.O[0][k] = OC; w.O[1][k] = OE; w.O[2][k] = OB; w.O[3][k] = OA;
+        w.O[4][k] = OTx; w.O[5][k] = OTz; w.O[6][k] = OWz; w.O[7][k] = OWx;
+    }
+    const float Kx = (KX - FX) - OC + OE, Kz = (KZ - FZ) - OB + OA;
+    const float hx = Wz - OWz, hz = -Wx + OWx;
+    const float dt2 = g.dt2;
+    const int wb = 1 - a.rb;
+    float d;
+    d = w.DV[0][k] + dt2 * ((Tx - OTx) + hx + 2.f * Kx); w.DV[0][k] = d; const float vx = w.V[a.rb][0][k] + d;
+    d = w.DV[1][k] + dt2 * ((Tz - OTz) + hz + 2.f * Kz); w.DV[1][k] = d; const float vz = w.V[a.rb][1][k] + d;
+    w.V[wb][0][k] = vx;
+    w.V[wb][1][k] = vz;
+    d = w.DS[0][k] + dt2 * (hx + Kx); w.DS[0][k] = d; const float sx = w.S[0][k] + d; w.S[0][k] = sx;
+    d = w.DS[1][k] + dt2 * (hz + Kz); w.DS[1][k] = d; const float sz = w.S[1][k] + d; w.S[1][k] = sz;
+    record(g, a, i, j, vx, vz, sx, sz);
+    correlate(g, a, k, i, j, vx, vz, sx, sz);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (reshot_u + reshot_s, P-S-seperating.cu:890-1042) fused:
+// on the interior, V and the stress-like fields of ring cells come from the
+// strips; the reverse step uses theta^{t+1} recomputed from V^{t+1}.  Writes the
+// source P velocity of the new level for the imaging condition.
+// ---------------------------------------------------------------------------
+// Stress-like fields at (i, j) for the reverse step: strips on the ring,
+// computed from the reconstructed totals elsewhere.
+template <int N>
+__device__ __forceinline__ void recon_stress_at(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s,
+                                                int rb, long so, const float *__restrict__ p2,
+                                                const float *__restrict__ q2, int i, int j, const FStep &a,
+                                                float &th, float &om, float &gxx, float &gxz, float &gzx,
+                                                float &gzz) {
+    long r = ring_index(rc.ring, i, j);
+    if (r >= 0) {
+        th = strip_get(s, rc.slot_th, RS_TH, r);
+        om = strip_get(s, rc.slot_th, RS_OM, r);
+        gzx = strip_get(s, rc.slot_th, RS_GZX, r);
+        gzz = strip_get(s, rc.slot_th, RS_GZZ, r);
+        gxx = strip_get(s, rc.slot_th, RS_GXX, r);
+        gxz = strip_get(s, rc.slot_th, RS_GXZ, r);
+        return;
+    }
+    const float *c = g.c;
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, a0, b0, a1, b1;
+#pragma unroll
+    for (int mm = 1; mm <= N; ++mm) {
+        recon_vin(g, w, rc, s, rb, so, i + mm - 1, j, a0, b0);
+        recon_vin(g, w, rc, s, rb, so, i - mm, j, a1, b1);
+        s1 += c[mm - 1] * (a0 - a1);
+        recon_vin(g, w, rc, s, rb, so, i + mm, j, a0, b0);
+        recon_vin(g, w, rc, s, rb, so, i - mm + 1, j, a1, b1);
+        s2 += c[mm - 1] * (b0 - b1);
+        recon_vin(g, w, rc, s, rb, so, i, j + mm, a0, b0);
+        recon_vin(g, w, rc, s, rb, so, i, j - mm + 1, a1, b1);
+        s3 += c[mm - 1] * (a0 - a1);
+        recon_vin(g, w, rc, s, rb, so, i, j + mm - 1, a0, b0);
+        recon_vin(g, w, rc, s, rb, so, i, j - mm, a1, b1);
+        s4 += c[mm - 1] * (b0 - b1);
+    }
+    gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
+    const long k0 = (long)i * g.P + j;
+    th = p2[k0] * (gxx + gzz);
+    if (rc.exact) th += src_term(g, a, i, j);
+    om = q2[k0] * (gxz - gzx);
+}
+
+// Stores stress-like fields of the ring frame and interior for the reverse
+// step of the next launch (the stress is a function of V^{t+1} only).
+template <int N>
+__global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const float *__restrict__ p2,
+                                                        const float *__restrict__ q2, FRecon rc, FStep a,
+                                                        const int2 *__restrict__ tiles) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (i >= g.Xn - g.L + N || j >= g.Zn - g.L + N) return;
+    // only the band next to the ring and its N-cell margin: the recon core forms its own
+    if (i >= g.L + 3 * N && i < g.Xn - g.L - 3 * N && j >= g.L + 3 * N && j < g.Zn - g.L - 3 * N) return;
+    const long so = (long)blockIdx.z * g.plane;
+    const KStrips s = shot_strips(rc);
+    float th, om, gxx, gxz, gzx, gzz;
+    // outer ring cells have no interior neighbours to differentiate; the
+    // reconstruction reads them only as strip values
+    if (ring_index(rc.ring, i, j) < 0 && (i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L)) return;
+    recon_stress_at<N>(g, w, rc, s, a.rb, so, p2, q2, i, j, a, th, om, gxx, gxz, gzx, gzz);
+    const long k = so + (long)i * g.P + j;
+    w.th[k] = th; w.om[k] = om; w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
+}
+
+// Reverse velocity step on the interior from the stored stress-like fields.
+template <int N>
+__global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, const float *__restrict__ q2, FRecon rc,
+                                                          FStep a, const int2 *__restrict__ tiles) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    const int L = g.L;
+    if (i < L || j < L || i >= g.Xn - L || j >= g.Zn - L) return;
+    if (i >= L + 2 * N && i < g.Xn - L - 2 * N && j >= L + 2 * N && j < g.Zn - L - 2 * N) return; // recon core
+    const long so = (long)blockIdx.z * g.plane;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    const float *c = g.c;
+    const int P = g.P;
+    float Wx = 0.f, Wz = 0.f, Tx = 0.f, Tz = 0.f, KX = 0.f, KZ = 0.f;
+    const bool safe = i >= L + 2 * N && i < g.Xn - L - 2 * N && j >= L + 2 * N && j < g.Zn - L - 2 * N;
+    const float q0 = safe ? q2[k0] : 0.f;
+#pragma unroll
+    for (int mm = 1; mm <= N; ++mm) {
+        const float cm = c[mm - 1];
+        const long xp = (long)mm * P, xm1 = (long)(mm - 1) * P;
+        Wx += cm * (w.om[k + xm1] - w.om[k - xp]);
+        Wz += cm * (w.om[k + mm - 1] - w.om[k - mm]);
+        Tx += cm * (w.th[k + xp] - w.th[k - xm1]);
+        Tz += cm * (w.th[k + mm] - w.th[k - mm + 1]);
+        KX += cm * ((w.gzx[k + mm - 1] * (q2[k0 + mm - 1] - q0) - w.gzx[k - mm] * (q2[k0 - mm] - q0)) * g.idz -
+                    (w.gzz[k + xp] * (q2[k0 + xp] - q0) - w.gzz[k - xm1] * (q2[k0 - xm1] - q0)) * g.idx);
+        KZ += cm * ((w.gxz[k + xm1] * (q2[k0 + xm1] - q0) - w.gxz[k - xp] * (q2[k0 - xp] - q0)) * g.idx -
+                    (w.gxx[k + mm] * (q2[k0 + mm] - q0) - w.gxx[k - mm + 1] * (q2[k0 - mm + 1] - q0)) * g.idz);
+    }
+    Wx *= g.idx; Wz *= g.idz; Tx *= g.idx; Tz *= g.idz;
+    const float dt2 = g.dt2;
+    const int wb = 1 - a.rb;
+    float d;
+    d = w.DV[0][k] - dt2 * (Tx + Wz + 2.f * KX); w.DV[0][k] = d; const float vx = w.V[a.rb][0][k] - d;
+    d = w.DV[1][k] - dt2 * (Tz - Wx + 2.f * KZ); w.DV[1][k] = d; const float vz = w.V[a.rb][1][k] - d;
+    w.V[wb][0][k] = vx;
+    w.V[wb][1][k] = vz;
+    d = w.DS[0][k] - dt2 * (Wz + KX); w.DS[0][k] = d; const float sx = w.S[0][k] - d; w.S[0][k] = sx;
+    d = w.DS[1][k] - dt2 * (-Wx + KZ); w.DS[1][k] = d; const float sz = w.S[1][k] - d; w.S[1][k] = sz;
+    rc.vpx[k] = vx - sx;
+    rc.vpz[k] = vz - sz;
+}
+
+// Strip store from the stored band stress and the written V (forward).
+__global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, long shot_strip,
+                                                     long shot_chunks, int slot, int vbuf, int with_v) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = q < r.ncells;
+    int i = r.L, j = r.L;
+    if (valid) ring_cell(r, q, i, j);
+    KStrips s = s0;
+    const long so = (long)blockIdx.z * g.plane;
+    if (s.f32) s.f32 += (long)blockIdx.z * shot_strip;
+    if (s.f16) s.f16 += (long)blockIdx.z * shot_strip;
+    if (s.e8) s.e8 += (long)blockIdx.z * shot_chunks;
+    const long k = so + (long)i * g.P + j;
+    if (with_v) {
+        strip_put(s, slot, RS_VX, q, valid, valid ? w.V[vbuf][0][k] : 0.f);
+        strip_put(s, slot, RS_VZ, q, valid, valid ? w.V[vbuf][1][k] : 0.f);
+    } else {
+        strip_put(s, slot, RS_TH, q, valid, valid ? w.th[k] : 0.f);
+        strip_put(s, slot, RS_OM, q, valid, valid ? w.om[k] : 0.f);
+        strip_put(s, slot, RS_GZX, q, valid, valid ? w.gzx[k] - w.F[3][k] : 0.f);
+        strip_put(s, slot, RS_GZZ, q, valid, valid ? w.gzz[k] - w.F[1][k] : 0.f);
+        strip_put(s, slot, RS_GXX, q, valid, valid ? w.gxx[k] - w.F[0][k] : 0.f);
+        strip_put(s, slot, RS_GXZ, q, valid, valid ? w.gxz[k] - w.F[2][k] : 0.f);
+    }
+}
+
+// Hand-off: (V^nt, D) -> (V^{nt-1}, D) into the other V buffer; P velocity
+// of levels nt and nt-1.
+__global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float *__restrict__ vp_nt_x,
+                                                 float *__restrict__ vp_nt_z, float *__restrict__ vp_m1_x,
+                                                 float *__restrict__ vp_m1_z) {
+    const int j = blockIdx.x * 64 + threadIdx.x, i = blockIdx.y * 4 + threadIdx.y;
+    if (i >= g.Xn || j >= g.Zn) return;
+    const long k = (long)blockIdx.z * g.plane + (long)i * g.P + j;
+    const float vx = w.V[rb][0][k], vz = w.V[rb][1][k], sx = w.S[0][k], sz = w.S[1][k];
+    vp_nt_x[k] = vx - sx;
+    vp_nt_z[k] = vz - sz;
+    const float vx1 = vx - w.DV[0][k], vz1 = vz - w.DV[1][k], sx1 = sx - w.DS[0][k], sz1 = sz - w.DS[1][k];
+    w.V[1 - rb][0][k] = vx1;
+    w.V[1 - rb][1][k] = vz1;
+    w.S[0][k] = sx1;
+    w.S[1][k] = sz1;
+    vp_m1_x[k] = vx1 - sx1;
+    vp_m1_z[k] = vz1 - sz1;
+}
+
+} // namespace ouc
