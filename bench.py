@@ -40,7 +40,28 @@ def parse():
     ap.add_argument("--cpu-sample-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib, reps=10):
+    """A plain device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters (expected: mib
+    MiB read and mib MiB written per launch)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "bytes_read_per_launch": n * 4,
+                      "bytes_written_per_launch": n * 4, "gbs": 2 * n * 4 * reps / dt / 1e9}))
 
 
 def main():
@@ -49,6 +70,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
+        return
     from ouc_amd import abi, velocity_models as vm
     from ouc_amd.engine import Engine, OucError
 
@@ -96,7 +120,7 @@ def main():
         "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
         "bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_ms, "launches_timed": klaunch,
     }
-    traffic = bytes_model.measured_traffic(kernel_name)
+    traffic = bytes_model.measured_traffic(kernel_name, batch if engine == abi.ENGINE_FUSED else 1)
     if traffic is not None:
         roofline["traffic"] = traffic
 

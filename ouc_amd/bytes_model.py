@@ -46,9 +46,11 @@ def dominant_kernel_bytes(p, engine, nshots):
     return plain_velocity_bytes(p), "k_velocity"
 
 
-def measured_traffic(kernel_name):
-    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 PMC
-    summary (profiles/pmc_traffic.md: lines 'kernel: bytes'), or None."""
+def measured_traffic(kernel_name, nshots=1):
+    """HBM bytes per launch of `kernel_name` over `nshots` shots, from the
+    committed rocprofv3 PMC summary (profiles/pmc_traffic.md: lines
+    'kernel: <bytes> bytes per shot', FETCH_SIZE and WRITE_SIZE already scaled
+    by a plain copy's readings), or None."""
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles", "pmc_traffic.md")
     if not os.path.exists(path):
         return None
@@ -57,7 +59,7 @@ def measured_traffic(kernel_name):
             line = line.strip().lstrip("-* ")
             if line.startswith(kernel_name + ":"):
                 try:
-                    return float(line.split(":", 1)[1].split()[0])
+                    return float(line.split(":", 1)[1].split()[0]) * nshots
                 except ValueError:
                     return None
     return None
