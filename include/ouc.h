@@ -123,6 +123,10 @@ typedef struct ouc_timings {
     double cell_updates;     /* padded-cell x step x wavefield updates performed */
     double canonical_bytes;  /* reference two-pass compulsory bytes for the same work */
     double own_bytes;        /* compulsory bytes of the kernels this engine ran */
+    int32_t fallback_shots;  /* shots re-run with CPML after the learned boundary diverged */
+    int32_t reserved;
+    double boundary_ratio;   /* learned boundary: largest |V| in the pad over the largest |V| in the
+                                band of the same width inside it, over shots and wavefields */
 } ouc_timings;
 
 typedef struct ouc_ctx ouc_ctx;
@@ -183,6 +187,49 @@ ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_
  * exponent.  No counterpart in the reference, whose strips are FP32
  * (P-S-seperating.cu:1852-1893). */
 ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, uint16_t *half_bits, int8_t *exponents);
+
+/* Learned absorbing boundary (boundary LEARNED / LEARNED_FALLBACK; no
+ * counterpart in the reference, whose only boundary is the CPML of
+ * P-S-seperating.cu:2067-2105).  The outer `band` cells run second-order
+ * staggered differences; a three-layer 3x3 convolutional network (5 -> 16 ->
+ * 16 -> 2 channels, linear, BF16 weights, FP32 accumulation) sets the outermost
+ * ring from the local velocity increment and stress.  weights: FP32 masters,
+ * n = 3312 (W1 [16][5][3][3], W2 [16][16][3][3], W3 [2][16][3][3]), rounded to
+ * BF16 here; NULL holds the ring at zero (the pad without the network).
+ * A context with the learned boundary starts without a network. */
+ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n);
+
+/* The composed 7x7 filter [2][5][7][7] the ring kernel applies (test hook). */
+ouc_status ouc_get_learned_filter(ouc_ctx *c, float *taps);
+
+/* Forward run of one shot that copies V (vx then vz) over the padded-grid
+ * rectangle [x0, x0+w) x [z0, z0+h) after each listed step (increasing) into
+ * out[n][2][w][h].  This is how the engine generates the wavefields its
+ * learned boundary trains on; no counterpart in the reference. */
+ouc_status ouc_forward_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const int32_t *steps, int32_t x0,
+                                 int32_t z0, int32_t w, int32_t h, float *out);
+
+/* RTM of one shot that copies the reconstructed source V^t (vx then vz) over
+ * the padded-grid rectangle at each listed time (decreasing, in [0, nt-2])
+ * into out[n][2][w][h].  Against ouc_forward_snapshots (V^t is the snapshot
+ * after forward step t-1) it measures the time reversibility of the
+ * reconstruction (reshot_u / reshot_s, P-S-seperating.cu:890-1042). */
+ouc_status ouc_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const int32_t *steps, int32_t x0,
+                               int32_t z0, int32_t w, int32_t h, float *out);
+
+/* Training of the learned-boundary network on the GPU (no context).  Windows
+ * x [M][5][H][W], targets y [M][2][H][W], mask [M][H][W] (1 where the loss
+ * counts), weights as in ouc_set_learned_boundary.  Loss = sum mask (out -
+ * y)^2 / (2 sum mask); the forward pass uses the BF16-rounded weights, and the
+ * gradient is taken with respect to them. */
+ouc_status ouc_lb_forward(int32_t device, int32_t M, int32_t H, int32_t W, const float *x, const float *weights,
+                          float *out);
+ouc_status ouc_lb_loss_grad(int32_t device, int32_t M, int32_t H, int32_t W, const float *x, const float *y,
+                            const float *mask, const float *weights, float *loss, float *grad);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over `epochs` full-batch steps; weights
+ * in/out; history: optional per-epoch losses. */
+ouc_status ouc_lb_train(int32_t device, int32_t M, int32_t H, int32_t W, const float *x, const float *y,
+                        const float *mask, int32_t epochs, float lr, float *weights, float *history);
 
 /* Message of the last failed call on c (or of the last failed ouc_create when
  * c is NULL).  Never NULL. */

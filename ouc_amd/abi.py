@@ -60,6 +60,7 @@ class Timings(C.Structure):
         ("image_ms", C.c_double), ("step_launches", C.c_int64), ("step_kernel_ms", C.c_double),
         ("step_kernel_launches", C.c_int64), ("cell_updates", C.c_double),
         ("canonical_bytes", C.c_double), ("own_bytes", C.c_double),
+        ("fallback_shots", C.c_int32), ("reserved", C.c_int32), ("boundary_ratio", C.c_double),
     ]
 
 

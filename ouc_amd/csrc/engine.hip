@@ -130,6 +130,12 @@ ouc_status fail(ouc_ctx *c, ouc_status code, const std::string &msg) {
     return code;
 }
 
+} // namespace
+
+ouc_status ouc::ouc_fail_global(ouc_status code, const std::string &msg) { return fail(nullptr, code, msg); }
+
+namespace {
+
 template <class F>
 ouc_status guarded(ouc_ctx *c, F &&f) {
     try {
@@ -327,6 +333,21 @@ struct ByteModel {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// The CPML context behind a learned-boundary context: same interior model,
+// shot geometry and time stepping, CPML pad of p.npml cells.
+ouc_ctx *fallback_ctx(ouc_ctx *c) {
+    if (c->fallback) return c->fallback;
+    ouc_params q = c->p;
+    q.boundary = OUC_BOUNDARY_CPML;
+    ouc_model m{c->vp_in.data(), c->vs_in.data(), nullptr};
+    ouc_ctx *fb = nullptr;
+    ouc_status st = ouc_create(&q, &m, c->device, &fb);
+    if (st != OUC_OK) throw Error(st, std::string("CPML fallback context: ") + ouc_last_error(nullptr));
+    fb->profiling = c->profiling;
+    c->fallback = fb;
+    return fb;
+}
+
 extern "C" {
 
 ouc_status ouc_create(const ouc_params *p, const ouc_model *m, int32_t device, ouc_ctx **out) {
@@ -357,6 +378,12 @@ ouc_status ouc_create(const ouc_params *p, const ouc_model *m, int32_t device, o
         pad_model(g, m->vp, vp);
         pad_model(g, m->vs, vs);
         cpml_profiles(g, p->R, vp, ax, az);
+        if (g.learned) {
+            std::fill(ax.begin(), ax.end(), 0.f);
+            std::fill(az.begin(), az.end(), 0.f);
+        }
+        c->vp_in.assign(m->vp, m->vp + (size_t)p->nx * p->nz);
+        c->vs_in.assign(m->vs, m->vs + (size_t)p->nx * p->nz);
         std::vector<float> p2(vp.size()), q2(vs.size());
         for (size_t k = 0; k < vp.size(); ++k) { p2[k] = vp[k] * vp[k]; q2[k] = vs[k] * vs[k]; }
         c->vp_pad = vp;
@@ -396,6 +423,10 @@ ouc_status ouc_forward_shot(ouc_ctx *c, const ouc_shot *s, float *seis) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        if (c->p.engine == OUC_ENGINE_FUSED) {
+            fused_forward_one(c, s, seis);
+            return;
+        }
         const Grid &g = c->g;
         std::vector<float> wav = shot_wavelet(c, s);
         c->last = ouc_timings{};
@@ -432,8 +463,8 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
-        if (c->p.engine == OUC_ENGINE_FUSED && !rec_in) {
-            fused_rtm_one(c, s, img_pp, img_ps, img_pp_lap, img_ps_lap);
+        if (c->p.engine == OUC_ENGINE_FUSED) {
+            fused_rtm_one(c, s, rec_in, img_pp, img_ps, img_pp_lap, img_ps_lap);
             return;
         }
         const Grid &g = c->g;
@@ -566,6 +597,39 @@ ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_
     return OUC_OK;
 }
 
+ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    return guarded(c, [&] {
+        if (!c->g.learned) throw Error(OUC_INVALID_ARG, "the context does not use the learned boundary");
+        fused_set_learned(c, weights, n);
+    });
+}
+
+ouc_status ouc_get_learned_filter(ouc_ctx *c, float *taps) {
+    if (!c || !taps) return fail(c, OUC_INVALID_ARG, "bad arguments");
+    return guarded(c, [&] { fused_get_learned_filter(c, taps); });
+}
+
+ouc_status ouc_forward_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const int32_t *steps, int32_t x0,
+                                 int32_t z0, int32_t w, int32_t h, float *out) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    return guarded(c, [&] {
+        check_shot(c, s);
+        if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "snapshots need the fused engine");
+        fused_snapshots(c, s, n, steps, x0, z0, w, h, out);
+    });
+}
+
+ouc_status ouc_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const int32_t *steps, int32_t x0,
+                               int32_t z0, int32_t w, int32_t h, float *out) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    return guarded(c, [&] {
+        check_shot(c, s);
+        if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "snapshots need the fused engine");
+        fused_recon_snapshots(c, s, n, steps, x0, z0, w, h, out);
+    });
+}
+
 const char *ouc_last_error(const ouc_ctx *c) {
     if (c) return c->err.c_str();
     return g_create_error.c_str();
@@ -573,6 +637,7 @@ const char *ouc_last_error(const ouc_ctx *c) {
 
 void ouc_destroy(ouc_ctx *c) {
     if (!c) return;
+    if (c->fallback) ouc_destroy(c->fallback);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     fused_destroy(c);
     lap_fft_destroy(c);

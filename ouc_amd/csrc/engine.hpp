@@ -21,6 +21,8 @@ struct Error : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char *what);
+// Records the message ouc_last_error(NULL) returns and passes the status on.
+ouc_status ouc_fail_global(ouc_status code, const std::string &msg);
 #define OUC_HIP(x) ::ouc::hip_check((x), #x)
 
 // Owning device allocation.
@@ -63,6 +65,8 @@ struct ouc_ctx {
     bool profiling = false;
 
     std::vector<float> vp_pad; // host copy (mute velocity)
+    std::vector<float> vp_in, vs_in; // interior model (the CPML fallback context is built from it)
+    ouc_ctx *fallback = nullptr;     // CPML context of a learned-boundary context, made on first need
     ouc::DevBuf p2, q2, ax, az;
     ouc::DevWave F, R;
     ouc::DevBuf vp3[3][2];

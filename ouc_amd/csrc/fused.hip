@@ -16,6 +16,7 @@
 #include "fused.hpp"
 #include "fused_kernels.hpp"
 #include "lapfft.hpp"
+#include "lpad_kernels.hpp"
 
 using namespace ouc;
 
@@ -51,7 +52,14 @@ struct FusedState {
     DevBuf tiles_s, tiles_v, tiles_rs, tiles_rv;
     int ntiles_s = 0, ntiles_v = 0, ntiles_rs = 0, ntiles_rv = 0;
     DevBuf img;            // [4][nx*nz] scratch for one shot's images
+    // learned boundary
+    DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
+    int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
+    bool lb_have = false;  // composed filter present (else the ring is held at zero)
+    DevBuf rec6;           // [6][nt][nx] forward seismograms of one shot
 };
+
+constexpr int kMonStride = 8; // steps between boundary-monitor launches
 
 FusedState *fs(ouc_ctx *c) { return (FusedState *)c->fused; }
 
@@ -77,6 +85,8 @@ FGrid make_fgrid(const ouc_ctx *c) {
     f.cx0 = g.L + g.N; f.cx1 = g.Xn - g.L - g.N;
     f.cz0 = g.L + g.N; f.cz1 = g.Zn - g.L - g.N;
     if (f.cx1 <= f.cx0 || f.cz1 <= f.cz0) { f.cx1 = f.cx0; f.cz1 = f.cz0; } // no core
+    f.lpad = g.learned ? g.L : 0;
+    f.vlo = g.learned ? 0 : g.N;
     return f;
 }
 
@@ -93,6 +103,45 @@ std::vector<int2> tile_list(const FGrid &g, Pred want) {
             if (any) out.push_back(make_int2(i0, j0));
         }
     return out;
+}
+
+// 64 x 4 tiles over the whole padded grid holding a cell within `depth` of the
+// edge.
+std::vector<int2> edge_tiles(const Grid &g, int depth) {
+    std::vector<int2> out;
+    for (int i0 = 0; i0 < g.Xn; i0 += 4)
+        for (int j0 = 0; j0 < g.Zn; j0 += 64) {
+            bool any = false;
+            for (int i = i0; i < std::min(i0 + 4, g.Xn) && !any; ++i)
+                for (int j = j0; j < std::min(j0 + 64, g.Zn) && !any; ++j)
+                    any = std::min(std::min(i, g.Xn - 1 - i), std::min(j, g.Zn - 1 - j)) < depth;
+            if (any) out.push_back(make_int2(i0, j0));
+        }
+    return out;
+}
+
+void build_learned(FusedState *st, const Grid &g) {
+    std::vector<int2> tp = edge_tiles(g, g.L), tm = edge_tiles(g, 2 * g.L);
+    st->ntiles_p = (int)tp.size();
+    st->ntiles_m = (int)tm.size();
+    st->tiles_p.alloc(tp.size() * sizeof(int2));
+    st->tiles_m.alloc(tm.size() * sizeof(int2));
+    OUC_HIP(hipMemcpy(st->tiles_p.ptr, tp.data(), tp.size() * sizeof(int2), hipMemcpyHostToDevice));
+    OUC_HIP(hipMemcpy(st->tiles_m.ptr, tm.data(), tm.size() * sizeof(int2), hipMemcpyHostToDevice));
+    // the network ring: velocity cells of the pad that the low-order scheme
+    // cannot update and that are not the always-zero ghost line
+    std::vector<LbCell> ring;
+    for (int i = 0; i < g.Xn; ++i)
+        for (int j = 0; j < g.Zn; ++j) {
+            const bool gx = i == g.Xn - 1, gz = j == g.Zn - 1;
+            const bool live_x = i >= 1 && i < g.Xn - 2 && j >= 1 && j < g.Zn - 1;
+            const bool live_z = i >= 1 && i < g.Xn - 1 && j >= 1 && j < g.Zn - 2;
+            if (!gx && !live_x) ring.push_back(LbCell{i, j, 0, 0});
+            if (!gz && !live_z) ring.push_back(LbCell{i, j, 1, 0});
+        }
+    st->nlbcells = (int)ring.size();
+    st->lbcells.alloc(ring.size() * sizeof(LbCell));
+    OUC_HIP(hipMemcpy(st->lbcells.ptr, ring.data(), ring.size() * sizeof(LbCell), hipMemcpyHostToDevice));
 }
 
 size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
@@ -174,6 +223,8 @@ void ensure(ouc_ctx *c, int shots) {
     st->tiles_rv.alloc(trv.size() * sizeof(int2));
     OUC_HIP(hipMemcpy(st->tiles_rs.ptr, trs.data(), trs.size() * sizeof(int2), hipMemcpyHostToDevice));
     OUC_HIP(hipMemcpy(st->tiles_rv.ptr, trv.data(), trv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    st->mon.alloc((size_t)2 * shots * 2 * sizeof(unsigned int)); // [pass][shot][pad, band]
+    if (g.learned) build_learned(st, g);
 }
 
 KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
@@ -252,10 +303,13 @@ struct CoreProbe {
 };
 
 void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, const FStep &a, CoreProbe &pr,
-                      int &nlaunch) {
+                      int &nlaunch, unsigned int *mon) {
     const FGrid &f = st->g;
     const KModel km = c->kmodel();
     hipStream_t s = c->stream;
+    const int B = (int)L.core.z;
+    if (f.lpad)
+        k_lpad_stress<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a);
     OUC_FDISPATCH(f.N, k_band_stress<NN><<<L.band_s, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_s.as<int2>(), a));
     if (L.has_core) {
         bool p = pr.want(nlaunch++);
@@ -264,11 +318,40 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
         if (p) { OUC_HIP(hipEventRecord(pr.ev[2 * pr.used + 1], s)); ++pr.used; }
     }
     OUC_FDISPATCH(f.N, k_band_velocity<NN><<<L.band_v, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_v.as<int2>(), a));
+    if (f.lpad) {
+        // the ring reads the pad's increments before the pad update advances them
+        const float sc = f.dt2 * f.idx;
+        k_lb_ring<<<dim3((st->nlbcells + 255) / 256, 1, B), 256, 0, s>>>(
+            f, w, km, st->lbcells.as<LbCell>(), st->nlbcells, st->lb_have ? st->lbw.f() : nullptr, a.rb, sc);
+        k_lpad_velocity<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a.rb);
+        if (a.t % kMonStride == 0)
+            k_lb_monitor<<<dim3(st->ntiles_m, 1, B), dim3(64, 4), 0, s>>>(f, w, st->tiles_m.as<int2>(), 1 - a.rb, mon);
+    }
 }
 
+// Crop of V (both components) of shot 0 into out[2][w][h] (x-major).
+__global__ void k_crop_v(FGrid g, const float *__restrict__ vx, const float *__restrict__ vz, int x0, int z0, int w,
+                         int h, float *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= h || i >= w) return;
+    const long k = (long)(x0 + i) * g.P + (z0 + j);
+    out[(long)i * h + j] = vx[k];
+    out[(long)w * h + (long)i * h + j] = vz[k];
+}
+
+// Wavefield snapshots taken after the listed forward steps (increasing).
+struct Snap {
+    const int *steps = nullptr;
+    int n = 0;
+    int x0 = 0, z0 = 0, w = 0, h = 0; // padded-grid rectangle
+    float *dev = nullptr;             // [n][2][w][h]
+};
+
 // Runs B shots through forward (+ strips) and, with rtm, the backward loop;
-// leaves the accumulators of every shot in st->acc.
-void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe) {
+// leaves the accumulators of every shot in st->acc.  With the learned
+// boundary, st->mon holds each shot's boundary monitor afterwards.
+void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe,
+               const Snap *snap = nullptr, const float *rec_in = nullptr, const Snap *rsnap = nullptr) {
     FusedState *st = fs(c);
     const Grid &g = c->g;
     const FGrid &f = st->g;
@@ -302,6 +385,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     OUC_HIP(hipMemcpyAsync(st->counts.ptr, counts.data(), counts.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     const size_t span = (size_t)st->plane * B * sizeof(float);
     OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
+    OUC_HIP(hipMemsetAsync(st->mon.ptr, 0, st->mon.bytes, s));
+    unsigned int *mon_f = st->mon.as<unsigned int>(), *mon_r = mon_f + 2 * st->cap;
     const KRing kr = c->kring();
     const KStrips ks0 = base_strips(st, c);
     const int rblocks = (int)((c->ring.ncells + 255) / 256);
@@ -312,13 +397,19 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     a.src = st->srcpos.as<int>();
     a.wav = st->wav.f();
     a.zr = zr;
-    a.rec = st->rec.f();
+    a.rec = ncomp == 6 ? st->rec6.f() : st->rec.f();
     a.ncomp = ncomp;
-    int rb = 0;
+    int rb = 0, ksnap = 0;
     for (int t = 0; t < nt; ++t) {
         a.t = t;
         a.rb = rb;
-        wave_step_probed(c, st, L, st->S, a, probe, nlaunch);
+        wave_step_probed(c, st, L, st->S, a, probe, nlaunch, mon_f);
+        if (snap && ksnap < snap->n && snap->steps[ksnap] == t) {
+            k_crop_v<<<dim3((snap->h + 255) / 256, snap->w), 256, 0, s>>>(
+                f, st->S.V[1 - rb][0], st->S.V[1 - rb][1], snap->x0, snap->z0, snap->w, snap->h,
+                snap->dev + (size_t)ksnap * 2 * snap->w * snap->h);
+            ++ksnap;
+        }
         if (rtm) {
             k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks, t, 0,
                                                                0);
@@ -330,6 +421,15 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     }
     OUC_HIP(hipGetLastError());
     if (!rtm) return;
+    if (rec_in) {
+        // observed records [2][nx][nt] (receiver-major) replace the modelled ones of shot 0
+        std::vector<float> tr((size_t)2 * nt * g.nx);
+        for (int q = 0; q < 2; ++q)
+            for (int ix = 0; ix < g.nx; ++ix)
+                for (int t = 0; t < nt; ++t) tr[((size_t)q * nt + t) * g.nx + ix] = rec_in[((size_t)q * g.nx + ix) * nt + t];
+        OUC_HIP(hipMemcpyAsync(st->rec.ptr, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        OUC_HIP(hipStreamSynchronize(s));
+    }
     c->last.forward_ms = 0; // filled by the caller from events
     OUC_HIP(hipEventRecord(c->ev[1], s));
     {
@@ -342,6 +442,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     int rbs = 1 - rb;
     OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
     for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, span, s));
+    int krs = 0;
     FStep r{};
     r.zr = zr;
     r.inj = st->rec.f();
@@ -374,6 +475,12 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, s>>>(f, st->S, c->q2.f(), rc, ra,
                                                                                         st->tiles_rv.as<int2>()));
             rbs = 1 - rbs;
+            if (rsnap && krs < rsnap->n && rsnap->steps[krs] == t) { // reconstructed V^t
+                k_crop_v<<<dim3((rsnap->h + 255) / 256, rsnap->w), 256, 0, s>>>(
+                    f, st->S.V[rbs][0], st->S.V[rbs][1], rsnap->x0, rsnap->z0, rsnap->w, rsnap->h,
+                    rsnap->dev + (size_t)krs * 2 * rsnap->w * rsnap->h);
+                ++krs;
+            }
         }
         const int lev = exact ? t + 1 : std::min(t + 2, nt);
         r.t = t;
@@ -382,7 +489,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         r.spz = st->vp[lev % 3][1].f();
         CoreProbe none;
         int nl2 = 0;
-        wave_step_probed(c, st, L, st->R, r, none, nl2);
+        wave_step_probed(c, st, L, st->R, r, none, nl2, mon_r);
         rbr = 1 - rbr;
     }
     OUC_HIP(hipGetLastError());
@@ -441,6 +548,57 @@ double fused_own_bytes(const ouc_ctx *c) {
     return (double)g.nt * (fwd + rcv + rec);
 }
 
+// Boundary verdict per shot of the last run_batch: a shot diverged when its
+// pad holds a non-finite value or more than kGrowth times the largest
+// amplitude that crossed the band just inside the pad (waves reach the pad only
+// through that band, so a stable boundary stays below it).
+constexpr float kGrowth = 10.f;
+
+std::vector<char> monitor_verdict(ouc_ctx *c, int B) {
+    std::vector<char> bad(B, 0);
+    if (!c->g.learned) return bad;
+    FusedState *st = fs(c);
+    std::vector<unsigned int> m((size_t)4 * st->cap);
+    OUC_HIP(hipMemcpyAsync(m.data(), st->mon.ptr, m.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+    double worst = 0;
+    for (int b = 0; b < B; ++b)
+        for (int pass = 0; pass < 2; ++pass) {
+            float pad, band;
+            std::memcpy(&pad, &m[(size_t)2 * (pass * st->cap + b)], 4);
+            std::memcpy(&band, &m[(size_t)2 * (pass * st->cap + b) + 1], 4);
+            const bool ok = std::isfinite(pad) && std::isfinite(band) && pad <= kGrowth * band;
+            if (!ok) bad[b] = 1;
+            const double ratio = band > 0 ? (double)pad / band : (pad > 0 ? INFINITY : 0.0);
+            worst = std::max(worst, std::isfinite(pad) ? ratio : INFINITY);
+        }
+    c->last.boundary_ratio = std::max(c->last.boundary_ratio, worst);
+    if (c->p.boundary == OUC_BOUNDARY_LEARNED)
+        for (int b = 0; b < B; ++b)
+            if (bad[b]) throw Error(OUC_DEVICE, "learned boundary diverged (no CPML fallback requested)");
+    return bad;
+}
+
+// Re-runs the listed shots on the CPML fallback context and adds their
+// Laplacian images to the host stacks; returns its device time.
+double rerun_with_cpml(ouc_ctx *c, const ouc_shot *shots, const std::vector<int> &idx, float *stack_pp,
+                       float *stack_ps, float *per_shot_ms, double t0_ms) {
+    ouc_ctx *fb = fallback_ctx(c);
+    const size_t ni = (size_t)c->g.nx * c->g.nz;
+    std::vector<ouc_shot> sub;
+    for (int k : idx) sub.push_back(shots[k]);
+    std::vector<float> pp(ni), ps(ni), ms(sub.size());
+    ouc_status st = ouc_rtm_batch(fb, sub.data(), (int32_t)sub.size(), pp.data(), ps.data(), ms.data());
+    if (st != OUC_OK) throw Error(st, std::string("CPML fallback: ") + ouc_last_error(fb));
+    for (size_t a = 0; a < ni; ++a) {
+        if (stack_pp) stack_pp[a] += pp[a];
+        if (stack_ps) stack_ps[a] += ps[a];
+    }
+    if (per_shot_ms)
+        for (size_t k = 0; k < idx.size(); ++k) per_shot_ms[idx[k]] = (float)(t0_ms + ms[k]);
+    return fb->last.total_ms;
+}
+
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms) {
     const Grid &g = c->g;
@@ -454,6 +612,7 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
     ensure(c, cap);
     FusedState *st = fs(c);
     const size_t ni = (size_t)g.nx * g.nz;
+    std::vector<int> diverged; // shots whose learned boundary diverged
     DevBuf stack;
     stack.alloc(2 * ni * sizeof(float));
     OUC_HIP(hipMemsetAsync(stack.ptr, 0, 2 * ni * sizeof(float), c->stream));
@@ -467,7 +626,12 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
         OUC_HIP(hipEventRecord(c->ev[4], c->stream));
         run_batch(c, shots + b0, B, true, 2, probe);
         OUC_HIP(hipEventRecord(c->ev[2], c->stream));
+        std::vector<char> bad = monitor_verdict(c, B);
         for (int b = 0; b < B; ++b) {
+            if (bad[b]) {
+                diverged.push_back(b0 + b);
+                continue;
+            }
             shot_images(c, st, b);
             k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[2].f(), stack.f());
             k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[3].f(), stack.f() + ni);
@@ -490,7 +654,9 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
     OUC_HIP(hipStreamSynchronize(c->stream));
     float tot = 0;
     OUC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
-    c->last.total_ms = tot;
+    double fb_ms = 0;
+    if (!diverged.empty()) fb_ms = rerun_with_cpml(c, shots, diverged, stack_pp, stack_ps, per_shot_ms, tot);
+    c->last.total_ms = tot + fb_ms;
     c->last.forward_ms = fwd_ms;
     c->last.backward_ms = bwd_ms;
     c->last.image_ms = img_ms;
@@ -501,18 +667,32 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
     c->last.cell_updates = (double)nshots * g.nt * (2 * padded + interior);
     c->last.canonical_bytes = (double)nshots * g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
     c->last.own_bytes = nshots * fused_own_bytes(c);
+    if (!diverged.empty()) {
+        c->last.fallback_shots = (int32_t)diverged.size();
+        throw Error(OUC_NUMERIC_FALLBACK, std::to_string(diverged.size()) +
+                                              " shot(s) ran again with CPML after the learned boundary diverged");
+    }
 }
 
 // Single shot through the fused path (a batch of one), returning its images.
-void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, float *img_pp, float *img_ps, float *lap_pp, float *lap_ps) {
+void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *img_pp, float *img_ps, float *lap_pp,
+                   float *lap_ps) {
     ensure(c, std::max(1, fs(c)->cap));
     FusedState *st = fs(c);
     CoreProbe probe;
     probe.init(c->g.nt, c->profiling);
     c->last = ouc_timings{};
     OUC_HIP(hipEventRecord(c->ev[0], c->stream));
-    run_batch(c, s, 1, true, 2, probe);
+    run_batch(c, s, 1, true, 2, probe, nullptr, rec_in);
     OUC_HIP(hipEventRecord(c->ev[2], c->stream));
+    if (monitor_verdict(c, 1)[0]) {
+        ouc_ctx *fb = fallback_ctx(c);
+        ouc_status r = ouc_rtm_shot(fb, s, rec_in, img_pp, img_ps, lap_pp, lap_ps);
+        if (r != OUC_OK) throw Error(r, std::string("CPML fallback: ") + ouc_last_error(fb));
+        c->last = fb->last;
+        c->last.fallback_shots = 1;
+        throw Error(OUC_NUMERIC_FALLBACK, "the learned boundary diverged; the shot ran again with CPML");
+    }
     shot_images(c, st, 0);
     OUC_HIP(hipEventRecord(c->ev[3], c->stream));
     const size_t ni = (size_t)c->g.nx * c->g.nz;
@@ -534,4 +714,170 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, float *img_pp, float *img_ps, 
     c->last.cell_updates = (double)g.nt * (2 * padded + interior);
     c->last.canonical_bytes = (double)g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
     c->last.own_bytes = fused_own_bytes(c);
+}
+
+// Forward seismograms [6][nx][nt] of one shot through the fused path.
+void fused_forward_one(ouc_ctx *c, const ouc_shot *s, float *seis) {
+    ensure(c, std::max(1, fs(c)->cap));
+    FusedState *st = fs(c);
+    const Grid &g = c->g;
+    const size_t n6 = (size_t)6 * g.nt * g.nx;
+    if (st->rec6.bytes < n6 * sizeof(float)) st->rec6.alloc(n6 * sizeof(float));
+    OUC_HIP(hipMemsetAsync(st->rec6.ptr, 0, n6 * sizeof(float), c->stream));
+    CoreProbe probe;
+    probe.init(g.nt, c->profiling);
+    c->last = ouc_timings{};
+    OUC_HIP(hipEventRecord(c->ev[0], c->stream));
+    run_batch(c, s, 1, false, 6, probe);
+    OUC_HIP(hipEventRecord(c->ev[1], c->stream));
+    std::vector<float> tmp(n6);
+    OUC_HIP(hipMemcpyAsync(tmp.data(), st->rec6.ptr, n6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+    if (monitor_verdict(c, 1)[0]) {
+        ouc_ctx *fb = fallback_ctx(c);
+        ouc_status r = ouc_forward_shot(fb, s, seis);
+        if (r != OUC_OK) throw Error(r, std::string("CPML fallback: ") + ouc_last_error(fb));
+        c->last = fb->last;
+        c->last.fallback_shots = 1;
+        throw Error(OUC_NUMERIC_FALLBACK, "the learned boundary diverged; the shot ran again with CPML");
+    }
+    float fw = 0;
+    OUC_HIP(hipEventElapsedTime(&fw, c->ev[0], c->ev[1]));
+    c->last.forward_ms = fw;
+    c->last.total_ms = fw;
+    c->last.step_launches = g.nt;
+    c->last.step_kernel_ms = probe.total_ms();
+    c->last.step_kernel_launches = probe.used;
+    c->last.cell_updates = (double)g.nt * g.Xn * g.Zn;
+    if (s->mute) {
+        std::vector<int32_t> cnt;
+        mute_counts(g, *s, c->vp_pad[(size_t)(s->sx + g.L) * g.P + s->rec_z + g.L], cnt);
+        for (int q = 0; q < 6; ++q)
+            for (int ix = 0; ix < g.nx; ++ix)
+                for (int t = 0; t < std::min(cnt[ix], g.nt); ++t) tmp[((size_t)q * g.nt + t) * g.nx + ix] = 0.f;
+    }
+    if (seis)
+        for (int q = 0; q < 6; ++q)
+            for (int t = 0; t < g.nt; ++t)
+                for (int ix = 0; ix < g.nx; ++ix)
+                    seis[((size_t)q * g.nx + ix) * g.nt + t] = tmp[((size_t)q * g.nt + t) * g.nx + ix];
+}
+
+// Forward run of one shot with snapshots of V over a padded-grid rectangle
+// after each listed step: out[n][2][w][h].
+void fused_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *steps, int x0, int z0, int w, int h,
+                     float *out) {
+    const Grid &g = c->g;
+    if (n < 1 || !steps || !out || w < 1 || h < 1 || x0 < 0 || z0 < 0 || x0 + w > g.Xn || z0 + h > g.Zn)
+        throw Error(OUC_INVALID_ARG, "snapshot rectangle or step list invalid");
+    for (int k = 0; k < n; ++k)
+        if (steps[k] < 0 || steps[k] >= g.nt || (k > 0 && steps[k] <= steps[k - 1]))
+            throw Error(OUC_INVALID_ARG, "snapshot steps must increase within [0, nt)");
+    ensure(c, std::max(1, fs(c)->cap));
+    DevBuf buf;
+    buf.alloc((size_t)n * 2 * w * h * sizeof(float));
+    std::vector<int> st(steps, steps + n);
+    Snap sn;
+    sn.steps = st.data();
+    sn.n = n;
+    sn.x0 = x0; sn.z0 = z0; sn.w = w; sn.h = h;
+    sn.dev = buf.f();
+    CoreProbe probe;
+    c->last = ouc_timings{};
+    run_batch(c, s, 1, false, 2, probe, &sn);
+    OUC_HIP(hipMemcpyAsync(out, buf.ptr, buf.bytes, hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+}
+
+namespace {
+float bf16_round(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) { // inf or NaN: truncate, keep NaN quiet
+        u &= 0xffff0000u;
+        if (x != x) u |= 0x00400000u;
+    } else {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    }
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+} // namespace
+
+// Network weights: W1 [16][5][3][3], W2 [16][16][3][3], W3 [2][16][3][3]
+// (FP32 masters, rounded to BF16).  The layers are linear, so they compose
+// exactly into one 7x7 filter [2][5][7][7]; the composition runs in FP64 over
+// the BF16 values and is rounded once to FP32.  weights == NULL: no network.
+void fused_set_learned(ouc_ctx *c, const float *weights, int n) {
+    FusedState *st = fs(c);
+    if (!weights) {
+        st->lb_have = false;
+        return;
+    }
+    const int n1 = 16 * 5 * 9, n2 = 16 * 16 * 9, n3 = 2 * 16 * 9;
+    if (n != n1 + n2 + n3) throw Error(OUC_INVALID_ARG, "expected " + std::to_string(n1 + n2 + n3) + " weights");
+    std::vector<double> w(n);
+    for (int a = 0; a < n; ++a) {
+        if (!std::isfinite(weights[a])) throw Error(OUC_INVALID_ARG, "weights must be finite");
+        w[a] = bf16_round(weights[a]);
+    }
+    const double *W1 = w.data(), *W2 = W1 + n1, *W3 = W2 + n2;
+    // A[h2][ci][5][5] = sum_h1 W2[h2][h1] (*) W1[h1][ci]
+    std::vector<double> A((size_t)16 * 5 * 25, 0.0), K((size_t)2 * 5 * 49, 0.0);
+    for (int h2 = 0; h2 < 16; ++h2)
+        for (int h1 = 0; h1 < 16; ++h1)
+            for (int ci = 0; ci < 5; ++ci)
+                for (int y2 = 0; y2 < 3; ++y2)
+                    for (int x2 = 0; x2 < 3; ++x2)
+                        for (int y1 = 0; y1 < 3; ++y1)
+                            for (int x1 = 0; x1 < 3; ++x1)
+                                A[((size_t)h2 * 5 + ci) * 25 + (y2 + y1) * 5 + (x2 + x1)] +=
+                                    W2[(h2 * 16 + h1) * 9 + y2 * 3 + x2] * W1[(h1 * 5 + ci) * 9 + y1 * 3 + x1];
+    for (int co = 0; co < 2; ++co)
+        for (int h2 = 0; h2 < 16; ++h2)
+            for (int ci = 0; ci < 5; ++ci)
+                for (int y3 = 0; y3 < 3; ++y3)
+                    for (int x3 = 0; x3 < 3; ++x3)
+                        for (int y = 0; y < 5; ++y)
+                            for (int x = 0; x < 5; ++x)
+                                K[((size_t)co * 5 + ci) * 49 + (y3 + y) * 7 + (x3 + x)] +=
+                                    W3[(co * 16 + h2) * 9 + y3 * 3 + x3] * A[((size_t)h2 * 5 + ci) * 25 + y * 5 + x];
+    std::vector<float> kf(K.begin(), K.end());
+    if (!st->lbw.ptr) st->lbw.alloc(kLbTaps * sizeof(float));
+    OUC_HIP(hipMemcpy(st->lbw.ptr, kf.data(), kLbTaps * sizeof(float), hipMemcpyHostToDevice));
+    st->lb_have = true;
+}
+
+void fused_get_learned_filter(ouc_ctx *c, float *taps) {
+    FusedState *st = fs(c);
+    if (!st->lb_have) throw Error(OUC_INVALID_ARG, "no network weights set");
+    OUC_HIP(hipMemcpy(taps, st->lbw.ptr, kLbTaps * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+// RTM of one shot with snapshots of the reconstructed source V^t over a
+// padded-grid rectangle at the listed times (decreasing, each in [0, nt-2]).
+void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *steps, int x0, int z0, int w, int h,
+                           float *out) {
+    const Grid &g = c->g;
+    if (n < 1 || !steps || !out || w < 1 || h < 1 || x0 < 0 || z0 < 0 || x0 + w > g.Xn || z0 + h > g.Zn)
+        throw Error(OUC_INVALID_ARG, "snapshot rectangle or step list invalid");
+    for (int k = 0; k < n; ++k)
+        if (steps[k] < 0 || steps[k] > g.nt - 2 || (k > 0 && steps[k] >= steps[k - 1]))
+            throw Error(OUC_INVALID_ARG, "reconstruction steps must decrease within [0, nt-2]");
+    ensure(c, std::max(1, fs(c)->cap));
+    DevBuf buf;
+    buf.alloc((size_t)n * 2 * w * h * sizeof(float));
+    std::vector<int> st(steps, steps + n);
+    Snap sn;
+    sn.steps = st.data();
+    sn.n = n;
+    sn.x0 = x0; sn.z0 = z0; sn.w = w; sn.h = h;
+    sn.dev = buf.f();
+    CoreProbe probe;
+    c->last = ouc_timings{};
+    run_batch(c, s, 1, true, 2, probe, nullptr, nullptr, &sn);
+    OUC_HIP(hipMemcpyAsync(out, buf.ptr, buf.bytes, hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
 }

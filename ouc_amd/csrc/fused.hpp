@@ -10,7 +10,21 @@ struct ouc_ctx;
 void fused_init(ouc_ctx *c);
 void fused_destroy(ouc_ctx *c);
 // One RTM shot through the fused kernels (a batch of one); images to host.
-void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, float *img_pp, float *img_ps, float *lap_pp, float *lap_ps);
+void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *img_pp, float *img_ps, float *lap_pp,
+                   float *lap_ps);
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms);
 double fused_own_bytes(const ouc_ctx *c);
+// Forward seismograms [6][nx][nt] of one shot through the fused kernels.
+void fused_forward_one(ouc_ctx *c, const ouc_shot *s, float *seis);
+// Forward run with V snapshots over a padded-grid rectangle after each listed step.
+void fused_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *steps, int x0, int z0, int w, int h,
+                     float *out);
+// RTM with snapshots of the reconstructed source V^t at the listed (decreasing) times.
+void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *steps, int x0, int z0, int w, int h,
+                           float *out);
+// Learned-boundary network weights (NULL: the pad without the network).
+void fused_set_learned(ouc_ctx *c, const float *weights, int n);
+void fused_get_learned_filter(ouc_ctx *c, float *taps);
+// The CPML context a learned-boundary context falls back to (engine.hip).
+ouc_ctx *fallback_ctx(ouc_ctx *c);

@@ -36,7 +36,17 @@ struct FGrid {
     float idx, idz, dt, dt2;
     float c[8];
     int cx0, cx1, cz0, cz1; // core
+    int vlo;        // V is nonzero on [vlo, Xn-vlo) x [vlo, Zn-vlo): N with CPML, 0 with the learned pad
+    int lpad;       // learned boundary: width of the low-order pad (0: CPML)
 };
+
+// Distance of cell (i, j) from the nearest edge of the padded grid.
+__device__ __forceinline__ int edge_dist(const FGrid &g, int i, int j) {
+    return min(min(i, g.Xn - 1 - i), min(j, g.Zn - 1 - j));
+}
+__device__ __forceinline__ bool in_lpad(const FGrid &g, int i, int j) {
+    return g.lpad > 0 && edge_dist(g, i, j) < g.lpad;
+}
 
 // One wavefield for all shots of a batch; arrays are [shot][plane].
 struct FWave {
@@ -71,7 +81,7 @@ __device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const F
                                          float &vx, float &vz) {
     vx = 0.f;
     vz = 0.f;
-    if (!in_update(g, i, j)) return;
+    if (i < g.vlo || i >= g.Xn - g.vlo || j < g.vlo || j >= g.Zn - g.vlo) return;
     long k = so + (long)i * g.P + j;
     vx = w.V[a.rb][0][k];
     vz = w.V[a.rb][1][k];
@@ -157,7 +167,7 @@ __device__ __forceinline__ KStrips shot_strips(const FRecon &rc) {
 // Totals seen by the reconstruction's stress pass at (i, j).
 __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s, int rb,
                                           long so, int i, int j, float &vx, float &vz) {
-    if (!in_update(g, i, j)) { vx = vz = 0.f; return; }
+    if (i < g.vlo || i >= g.Xn - g.vlo || j < g.vlo || j >= g.Zn - g.vlo) { vx = vz = 0.f; return; }
     long r = rc.slot_v >= 0 ? ring_index(rc.ring, i, j) : -1;
     if (r >= 0) {
         vx = strip_get(s, rc.slot_v, RS_VX, r);
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m,
                                                      FStep a) {
     const int2 t0 = tiles[blockIdx.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
-    if (!in_update(g, i, j)) return;
+    if (!in_update(g, i, j) || in_lpad(g, i, j)) return;
     // skip core cells farther than N from the band
     if (i >= g.cx0 + N && i < g.cx1 - N && j >= g.cz0 + N && j < g.cz1 - N) return;
     const long so = (long)blockIdx.z * g.plane;
@@ -394,7 +404,7 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
                                                        FStep a) {
     const int2 t0 = tiles[blockIdx.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
-    if (!in_update(g, i, j) || in_core(g, i, j)) return;
+    if (!in_update(g, i, j) || in_core(g, i, j) || in_lpad(g, i, j)) return;
     const long so = (long)blockIdx.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const float *c = g.c;
@@ -405,7 +415,12 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
     // Cg - Eg and Bg - Ag in the product-rule form (q - q0).
     float Wx = 0.f, Wz = 0.f, Tx = 0.f, Tz = 0.f, KX = 0.f, KZ = 0.f;
     float Cg = 0.f, Eg = 0.f, Bg = 0.f, Ag = 0.f, Cf = 0.f, Ef = 0.f, Bf = 0.f, Af = 0.f;
-    const float q0 = q[k0];
+    // The product-rule form subtracts q0 (D-z gzx - D+x gzz), which vanishes
+    // because the staggered x and z operators commute -- only if every g came
+    // from the same operator.  Within N of the learned boundary's low-order pad
+    // the g's do not, so the plain form (q0 = 0) is used there, as the
+    // reconstruction uses it next to its ring.
+    const float q0 = (g.lpad && edge_dist(g, i, j) < g.lpad + N) ? 0.f : q[k0];
 #pragma unroll
     for (int mm = 1; mm <= N; ++mm) {
         const float cm = c[mm - 1];

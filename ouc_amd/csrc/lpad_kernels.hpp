@@ -1,0 +1,220 @@
+// Learned absorbing boundary: kernels of the low-order pad.
+//
+// The pad is the outer `lpad` cells of the padded grid on every side.  There
+// the scheme runs with second-order staggered differences (reach one cell), so
+// only the outermost ring of velocity cells lacks the data of a full update;
+// a small network supplies that ring from a 7x7 window of the local velocity
+// increment and stress.  Inside the pad the stress and velocity follow the
+// same equations as the interior (theta, omega and the product terms of the
+// reference's forward_s / forward_u, P-S-seperating.cu:205-384) without CPML.
+//
+// Liveness is mirror-symmetric on the staggered grid (x shown; z alike):
+//   theta-type stress (theta, gxx, gzz) at i in [1, Xn-1);
+//   omega-type stress (omega, gxz, gzx) at i in [0, Xn-1);
+//   vx (at i+1/2): scheme on [1, Xn-2), network at 0 and Xn-2, zero at Xn-1;
+//   vz (at i):     scheme on [1, Xn-1), network at 0 and Xn-1.
+// In the frame of each side (a: cells from the outer edge, b: along it) the
+// network sees the same geometry on all four sides.
+#pragma once
+
+#include "fused_kernels.hpp"
+
+namespace ouc {
+
+constexpr int kLbIn = 5, kLbOut = 2, kLbK = 7;     // composed filter: 2 x 5 x 7 x 7
+constexpr int kLbTaps = kLbOut * kLbIn * kLbK * kLbK;
+
+// Network ring entry: the velocity cell (i, j), field (0: vx, 1: vz).
+struct LbCell {
+    int i, j, f, pad;
+};
+
+// 2nd-order stress on pad cells (those with edge_dist < lpad).  V is read as
+// the interior's stress pass sees it (load_vin): on the receiver row of the
+// receiver wavefield that is the injected record.
+__global__ __launch_bounds__(256) void k_lpad_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                     FStep a) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (i >= g.Xn || j >= g.Zn || !in_lpad(g, i, j)) return;
+    const long so = (long)blockIdx.z * g.plane;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    const bool t_live = i >= 1 && i < g.Xn - 1 && j >= 1 && j < g.Zn - 1;
+    const bool w_live = i < g.Xn - 1 && j < g.Zn - 1;
+    float vx0, vz0, vxa, vza, vxb, vzb;
+    load_vin(g, w, a, so, i, j, vx0, vz0);
+    float gxx = 0.f, gzz = 0.f, gxz = 0.f, gzx = 0.f;
+    if (t_live) {
+        load_vin(g, w, a, so, i - 1, j, vxa, vza);
+        load_vin(g, w, a, so, i, j - 1, vxb, vzb);
+        gxx = (vx0 - vxa) * g.idx;
+        gzz = (vz0 - vzb) * g.idz;
+    }
+    if (w_live) {
+        load_vin(g, w, a, so, i, j + 1, vxa, vza);
+        load_vin(g, w, a, so, i + 1, j, vxb, vzb);
+        gxz = (vxa - vx0) * g.idz;
+        gzx = (vzb - vz0) * g.idx;
+    }
+    w.th[k] = m.p[k0] * (gxx + gzz);
+    w.om[k] = m.q[k0] * (gxz - gzx);
+    w.gxx[k] = gxx; w.gzz[k] = gzz; w.gxz[k] = gxz; w.gzx[k] = gzx;
+}
+
+// 2nd-order velocity update on the live pad cells (the network ring and the
+// ghost line are left to k_lb_ring / stay zero).  The P/S split is not carried
+// in the pad: nothing reads S there.
+__global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                       int rb) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (i >= g.Xn || j >= g.Zn || !in_lpad(g, i, j)) return;
+    const long so = (long)blockIdx.z * g.plane;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    const long P = g.P;
+    const float *q = m.q;
+    const int wb = 1 - rb;
+    const float dt2 = g.dt2;
+    if (i >= 1 && i < g.Xn - 2 && j >= 1 && j < g.Zn - 1) { // vx
+        const float Tx = (w.th[k + P] - w.th[k]) * g.idx;
+        const float Wz = (w.om[k] - w.om[k - 1]) * g.idz;
+        const float C = (q[k0] * w.gzx[k] - q[k0 - 1] * w.gzx[k - 1]) * g.idz;
+        const float E = (q[k0 + P] * w.gzz[k + P] - q[k0] * w.gzz[k]) * g.idx;
+        const float d = w.DV[0][k] + dt2 * (Tx + Wz + 2.f * (C - E));
+        w.DV[0][k] = d;
+        w.V[wb][0][k] = w.V[rb][0][k] + d;
+    }
+    if (i >= 1 && i < g.Xn - 1 && j >= 1 && j < g.Zn - 2) { // vz
+        const float Tz = (w.th[k + 1] - w.th[k]) * g.idz;
+        const float Wx = (w.om[k] - w.om[k - P]) * g.idx;
+        const float B = (q[k0] * w.gxz[k] - q[k0 - P] * w.gxz[k - P]) * g.idx;
+        const float A = (q[k0 + 1] * w.gxx[k + 1] - q[k0] * w.gxx[k]) * g.idz;
+        const float d = w.DV[1][k] + dt2 * (Tz - Wx + 2.f * (B - A));
+        w.DV[1][k] = d;
+        w.V[wb][1][k] = w.V[rb][1][k] + d;
+    }
+}
+
+// Input channel `ch` of the network (0: D_x, 1: D_z, 2: txx, 3: tzz, 4: txz,
+// stresses scaled by dt^2/dx) at original cell (i, j); zero outside the grid.
+// txx = theta - 2 q gzz, tzz = theta - 2 q gxx, txz = q (gxz + gzx).
+__device__ __forceinline__ float lb_input(const FGrid &g, const FWave &w, const float *__restrict__ q, long so, int ch,
+                                          int i, int j, float sc) {
+    if (i < 0 || i >= g.Xn || j < 0 || j >= g.Zn) return 0.f;
+    const long k0 = (long)i * g.P + j, k = so + k0;
+    switch (ch) {
+    case 0: return w.DV[0][k];
+    case 1: return w.DV[1][k];
+    case 2: return sc * (w.th[k] - 2.f * q[k0] * w.gzz[k]);
+    case 3: return sc * (w.th[k] - 2.f * q[k0] * w.gxx[k]);
+    default: return sc * q[k0] * (w.gxz[k] + w.gzx[k]);
+    }
+}
+
+// Field staggering (x, z) and mirror signs of the 5 channels, and the channel
+// each one becomes under the x <-> z transpose.
+__device__ __constant__ const int kLbSX[5] = {1, 0, 0, 0, 1};
+__device__ __constant__ const int kLbSZ[5] = {0, 1, 0, 0, 1};
+__device__ __constant__ const float kLbXS[5] = {-1.f, 1.f, 1.f, 1.f, -1.f};
+__device__ __constant__ const float kLbZS[5] = {1.f, -1.f, 1.f, 1.f, -1.f};
+__device__ __constant__ const int kLbT[5] = {1, 0, 3, 2, 4};
+
+// Canonical -> original cell of canonical channel cc at (a, b) on `side`
+// (0: x-low, 1: x-high, 2: z-low, 3: z-high); returns the original channel and
+// the sign.
+__device__ __forceinline__ int lb_map(const FGrid &g, int side, int cc, int a, int b, int &i, int &j, float &sg) {
+    if (side < 2) {
+        i = side == 0 ? a : g.Xn - 1 - kLbSX[cc] - a;
+        j = b;
+        sg = side == 0 ? 1.f : kLbXS[cc];
+        return cc;
+    }
+    const int oc = kLbT[cc];
+    j = side == 2 ? a : g.Zn - 1 - kLbSZ[oc] - a;
+    i = b;
+    sg = side == 2 ? 1.f : kLbZS[oc];
+    return oc;
+}
+
+// Network on the outer ring: for each ring cell, the side with the smallest
+// canonical depth (ties: x sides first, low before high), the 7x7x5 canonical
+// window, the composed filter, and the correction added to the increment:
+// D += delta, V_new = V + D.  With wts == NULL the ring is held at zero (the
+// pad without the network).
+__global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, const LbCell *__restrict__ cells,
+                                                 int ncells, const float *__restrict__ wts, int rb, float sc) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ncells) return;
+    const LbCell cl = cells[e];
+    const long so = (long)blockIdx.z * g.plane;
+    const long k = so + (long)cl.i * g.P + cl.j;
+    const int wb = 1 - rb;
+    if (!wts) {
+        w.DV[cl.f][k] = 0.f;
+        w.V[wb][cl.f][k] = 0.f;
+        return;
+    }
+    // canonical depth of this cell on each side
+    const int f = cl.f; // original channel 0 (vx) or 1 (vz)
+    int da[4];
+    da[0] = cl.i;
+    da[1] = g.Xn - 1 - kLbSX[f] - cl.i;
+    da[2] = cl.j;
+    da[3] = g.Zn - 1 - kLbSZ[f] - cl.j;
+    int side = 0;
+    for (int s2 = 1; s2 < 4; ++s2)
+        if (da[s2] < da[side]) side = s2;
+    const int a = da[side];
+    const int b = side < 2 ? cl.j : cl.i;
+    const int oc = side < 2 ? f : kLbT[f]; // canonical output channel of this field
+    float acc = 0.f;
+    const float *wo = wts + (long)oc * kLbIn * kLbK * kLbK;
+    for (int cc = 0; cc < kLbIn; ++cc) {
+        for (int dy = 0; dy < kLbK; ++dy) {
+            const int aa = a + dy - 3;
+            if (aa < 0) continue;
+            for (int dx = 0; dx < kLbK; ++dx) {
+                int i, j;
+                float sg;
+                const int ch = lb_map(g, side, cc, aa, b + dx - 3, i, j, sg);
+                acc += wo[(cc * kLbK + dy) * kLbK + dx] * sg * lb_input(g, w, m.q, so, ch, i, j, sc);
+            }
+        }
+    }
+    // back to the original field: canonical output oc maps to field f with sign
+    float sg_out = 1.f;
+    if (side == 1) sg_out = kLbXS[f];
+    if (side == 3) sg_out = kLbZS[f];
+    const float d = w.DV[f][k] + sg_out * acc;
+    w.DV[f][k] = d;
+    w.V[wb][f][k] = w.V[rb][f][k] + d;
+}
+
+// Largest |V| of each shot over the pad (mon[2 shot]) and over the band of the
+// same width just inside it (mon[2 shot + 1]), as running maxima.  Float bits
+// compare as integers: |x| has a clear sign bit and NaN sorts above +inf.
+__global__ __launch_bounds__(256) void k_lb_monitor(FGrid g, FWave w, const int2 *__restrict__ tiles, int rb,
+                                                    unsigned int *__restrict__ mon) {
+    const int2 t0 = tiles[blockIdx.x];
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    unsigned int up = 0u, uf = 0u;
+    if (i < g.Xn && j < g.Zn) {
+        const int d = edge_dist(g, i, j);
+        if (d < 2 * g.lpad) {
+            const long k = (long)blockIdx.z * g.plane + (long)i * g.P + j;
+            const unsigned int u = max(__float_as_uint(fabsf(w.V[rb][0][k])), __float_as_uint(fabsf(w.V[rb][1][k])));
+            if (d < g.lpad) up = u;
+            else uf = u;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        up = max(up, (unsigned int)__shfl_xor((int)up, o));
+        uf = max(uf, (unsigned int)__shfl_xor((int)uf, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(mon + 2 * blockIdx.z, up);
+        atomicMax(mon + 2 * blockIdx.z + 1, uf);
+    }
+}
+
+} // namespace ouc

@@ -21,9 +21,16 @@ std::string make_grid(const ouc_params *p, Grid &g) {
     if (p->boundary != OUC_BOUNDARY_CPML && p->boundary != OUC_BOUNDARY_LEARNED &&
         p->boundary != OUC_BOUNDARY_LEARNED_FALLBACK)
         return "unknown boundary";
-    int64_t xn = (int64_t)p->nx + 2 * p->npml, zn = (int64_t)p->nz + 2 * p->npml;
+    int pad = p->npml;
+    if (p->boundary != OUC_BOUNDARY_CPML) {
+        if (p->engine != OUC_ENGINE_FUSED) return "the learned boundary runs on the fused engine";
+        pad = p->band > 0 ? p->band : 8;
+        if (pad < p->half_order || pad < 4) return "band must be >= max(half_order, 4)";
+    }
+    int64_t xn = (int64_t)p->nx + 2 * pad, zn = (int64_t)p->nz + 2 * pad;
     if (xn * zn > (int64_t)1 << 31) return "grid too large";
-    g.nx = p->nx; g.nz = p->nz; g.L = p->npml; g.N = p->half_order;
+    g.nx = p->nx; g.nz = p->nz; g.L = pad; g.N = p->half_order;
+    g.learned = p->boundary != OUC_BOUNDARY_CPML;
     g.Xn = (int)xn; g.Zn = (int)zn;
     g.P = (g.Zn + 31) / 32 * 32;
     g.nt = p->nt;

@@ -18,6 +18,7 @@ struct Grid {
     int P = 0;           // device row pitch (floats), Zn rounded up
     int nt = 0;
     float dx = 0, dz = 0, dt = 0;
+    bool learned = false; // learned boundary: L is the low-order pad, no CPML damping
     size_t cells() const { return (size_t)Xn * Zn; }
     size_t pitched() const { return (size_t)Xn * P; }
 };

@@ -6,6 +6,7 @@ fallback: if the HIP library is missing, importing this module's loader raises.
 """
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
@@ -48,6 +49,21 @@ def lib():
     L.ouc_write_image_files.restype = C.c_int
     L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, P(C.c_uint16), P(C.c_int8)]
     L.ouc_strip_codec_fp16.restype = C.c_int
+    L.ouc_set_learned_boundary.argtypes = [C.c_void_p, fp, C.c_int32]
+    L.ouc_set_learned_boundary.restype = C.c_int
+    L.ouc_get_learned_filter.argtypes = [C.c_void_p, fp]
+    L.ouc_get_learned_filter.restype = C.c_int
+    L.ouc_forward_snapshots.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, P(C.c_int32), C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int32, fp]
+    L.ouc_forward_snapshots.restype = C.c_int
+    L.ouc_recon_snapshots.argtypes = L.ouc_forward_snapshots.argtypes
+    L.ouc_recon_snapshots.restype = C.c_int
+    L.ouc_lb_forward.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp]
+    L.ouc_lb_forward.restype = C.c_int
+    L.ouc_lb_loss_grad.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp, fp, fp, fp]
+    L.ouc_lb_loss_grad.restype = C.c_int
+    L.ouc_lb_train.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp, C.c_int32, C.c_float, fp, fp]
+    L.ouc_lb_train.restype = C.c_int
     L.ouc_last_error.argtypes = [C.c_void_p]
     L.ouc_last_error.restype = C.c_char_p
     L.ouc_destroy.argtypes = [C.c_void_p]
@@ -86,6 +102,14 @@ class Engine:
         self._h = h
 
     def _check(self, st):
+        """Raise on failure.  OUC_NUMERIC_FALLBACK is not a failure: the
+        learned boundary diverged and the results come from the CPML context;
+        it sets self.fell_back and warns."""
+        self.fell_back = False
+        if st == abi.OUC_NUMERIC_FALLBACK:
+            self.fell_back = True
+            warnings.warn("ouc: " + lib().ouc_last_error(self._h).decode(), RuntimeWarning)
+            return
         if st != abi.OUC_OK:
             raise OucError(st, lib().ouc_last_error(self._h).decode())
 
@@ -140,6 +164,42 @@ class Engine:
             out["per_shot_ms"] = ms
         return out
 
+    def set_learned_boundary(self, weights):
+        """FP32 master weights of the boundary network (3312 values), or None
+        for the pad without the network."""
+        if weights is None:
+            self._check(lib().ouc_set_learned_boundary(self._h, abi.fptr(None), 0))
+            return
+        w = np.ascontiguousarray(weights, np.float32).ravel()
+        self._check(lib().ouc_set_learned_boundary(self._h, abi.fptr(w), w.size))
+
+    def learned_filter(self):
+        """The composed filter [2][5][7][7] the ring kernel applies."""
+        out = np.zeros((2, 5, 7, 7), np.float32)
+        self._check(lib().ouc_get_learned_filter(self._h, abi.fptr(out)))
+        return out
+
+    def forward_snapshots(self, shot, steps, x0, z0, w, h):
+        """V (vx, vz) over the padded-grid rectangle after each listed step:
+        [len(steps), 2, w, h]."""
+        s, keep = _shot_struct(shot, self.p.nt)
+        st = np.ascontiguousarray(steps, np.int32)
+        out = np.zeros((st.size, 2, w, h), np.float32)
+        self._check(lib().ouc_forward_snapshots(self._h, C.byref(s), st.size,
+                                                st.ctypes.data_as(C.POINTER(C.c_int32)), int(x0), int(z0), int(w),
+                                                int(h), abi.fptr(out)))
+        return out
+
+    def recon_snapshots(self, shot, steps, x0, z0, w, h):
+        """Reconstructed source V^t over the padded-grid rectangle at each
+        listed (decreasing) time: [len(steps), 2, w, h]."""
+        s, keep = _shot_struct(shot, self.p.nt)
+        st = np.ascontiguousarray(steps, np.int32)
+        out = np.zeros((st.size, 2, w, h), np.float32)
+        self._check(lib().ouc_recon_snapshots(self._h, C.byref(s), st.size, st.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              int(x0), int(z0), int(w), int(h), abi.fptr(out)))
+        return out
+
     def timings(self):
         t = abi.Timings()
         self._check(lib().ouc_get_timings(self._h, C.byref(t)))
@@ -166,3 +226,46 @@ def strip_codec_fp16(x, device=0):
     if st != abi.OUC_OK:
         raise OucError(st, lib().ouc_last_error(None).decode())
     return h, e
+
+
+def _lb_check(st):
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
+
+
+def lb_forward(x, weights, device=0):
+    """Layered boundary network on the GPU: x [M][5][H][W] -> [M][2][H][W]."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(weights, np.float32).ravel()
+    M, Cn, H, W = x.shape
+    assert Cn == 5
+    out = np.zeros((M, 2, H, W), np.float32)
+    _lb_check(lib().ouc_lb_forward(int(device), M, H, W, abi.fptr(x), abi.fptr(w), abi.fptr(out)))
+    return out
+
+
+def lb_loss_grad(x, y, mask, weights, device=0):
+    """(loss, gradient) of the masked MSE of the boundary network on the GPU."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    mk = np.ascontiguousarray(mask, np.float32)
+    w = np.ascontiguousarray(weights, np.float32).ravel()
+    M, _, H, W = x.shape
+    loss = np.zeros(1, np.float32)
+    grad = np.zeros_like(w)
+    _lb_check(lib().ouc_lb_loss_grad(int(device), M, H, W, abi.fptr(x), abi.fptr(y), abi.fptr(mk), abi.fptr(w),
+                                     abi.fptr(loss), abi.fptr(grad)))
+    return float(loss[0]), grad
+
+
+def lb_train(x, y, mask, weights, epochs, lr, device=0):
+    """Adam on the GPU; returns (weights, per-epoch losses)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    mk = np.ascontiguousarray(mask, np.float32)
+    w = np.ascontiguousarray(weights, np.float32).ravel().copy()
+    M, _, H, W = x.shape
+    hist = np.zeros(int(epochs), np.float32)
+    _lb_check(lib().ouc_lb_train(int(device), M, H, W, abi.fptr(x), abi.fptr(y), abi.fptr(mk), int(epochs),
+                                 float(lr), abi.fptr(w), abi.fptr(hist)))
+    return w, hist

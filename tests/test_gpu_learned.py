@@ -1,0 +1,251 @@
+"""GPU tests of the learned absorbing boundary: training kernels against the
+FP64 oracle, the ring and pad kernels against the host-side definition the
+training data use, what the trained network gains over the same band without
+it, and the CPML fallback."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lb_net
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def LB():
+    from ouc_amd import learned
+
+    return learned
+
+
+@pytest.fixture(scope="module")
+def weights(LB):
+    assert os.path.exists(LB.WEIGHTS_PATH), "trained boundary weights are not committed"
+    return LB.load_weights()
+
+
+def test_loss_gradient_and_forward_match_fp64_oracle():
+    from ouc_amd import engine as E
+
+    rng = np.random.default_rng(0)
+    w = (0.3 * rng.standard_normal(lb_net.N_WEIGHTS)).astype(np.float32)
+    x = rng.standard_normal((4, 5, 7, 30)).astype(np.float32)
+    y = rng.standard_normal((4, 2, 7, 30)).astype(np.float32)
+    m = (rng.random((4, 7, 30)) > 0.5).astype(np.float32)
+    lg, gg = E.lb_loss_grad(x, y, m, w)
+    lo, go = lb_net.loss_grad(x, y, m, w)
+    print(f"loss rel err {abs(lg - lo) / lo:.2e}, gradient rel L2 {np.linalg.norm(gg - go) / np.linalg.norm(go):.2e}")
+    assert abs(lg - lo) <= 1e-5 * lo
+    assert np.linalg.norm(gg - go) <= 1e-5 * np.linalg.norm(go)
+    fo = E.lb_forward(x, w)
+    fr = lb_net.forward(x, w)[-1]
+    assert np.linalg.norm(fo - fr) <= 1e-5 * np.linalg.norm(fr)
+
+
+def test_training_reduces_loss_on_engine_windows(LB):
+    from ouc_amd import engine as E
+
+    c = LB.Case(48, 77, nt=260)
+    x, y, m = LB.case_windows(c, 12, 5e-3)
+    assert len(x) > 0
+    w0 = LB.init_weights(1)
+    # the gradient on real windows agrees with the oracle too
+    _, gg = E.lb_loss_grad(x[:16], y[:16], m[:16], w0)
+    _, go = lb_net.loss_grad(x[:16], y[:16], m[:16], w0)
+    assert np.linalg.norm(gg - go) <= 1e-4 * np.linalg.norm(go)
+    w, hist = LB.train(x, y, m, 300, 2e-3, w0=w0)
+    print(f"loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    assert hist[-1] < 0.5 * hist[0]
+
+
+def test_engine_composes_the_bf16_layers(LB, weights):
+    from ouc_amd import abi
+    from ouc_amd.engine import Engine
+
+    c = LB.Case(32, 5, nt=50)
+    with Engine(c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs) as e:
+        e.set_learned_boundary(weights)
+        K = e.learned_filter()
+    Ko = lb_net.composed(weights)
+    assert np.abs(K - Ko).max() <= 1e-6 * np.abs(Ko).max()
+
+
+def test_ring_and_pad_kernels_follow_the_training_definition(LB, weights):
+    """One engine step on the learned pad, recomputed on the host in FP64 from
+    the engine's own V^{n-1}, V^n: the ring from the composed filter on the
+    side frames (as the training windows see it), the live pad cells from the
+    second-order update."""
+    from ouc_amd import abi
+    from ouc_amd.engine import Engine
+
+    n, L = 40, 8
+    c = LB.Case(n, 11, nt=150)
+    X = n + 2 * L
+    t = 148
+    with Engine(c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs) as e:
+        e.set_learned_boundary(weights)
+        snaps = e.forward_snapshots(c.shot(), [t - 2, t - 1, t], 0, 0, X, X).astype(np.float64)
+        K = e.learned_filter().astype(np.float64)
+    Vm, V, Vn = snaps
+    p2 = LB.edge_pad(c.vp.astype(np.float64), L) ** 2
+    q2 = LB.edge_pad(c.vs.astype(np.float64), L) ** 2
+    ch = LB.pad_channels(V, Vm, p2, q2, c.dx, c.dx, c.dt)
+    assert np.abs(V[:, :L]).max() > 1e-6 * np.abs(V).max(), "the wave has not reached the pad"
+    # ring
+    exp, got = [], []
+    for side in range(4):
+        cs = LB.to_side(ch, side)
+        A, Bn = cs[0].shape
+        P = np.zeros((5, A + 6, Bn + 6))
+        for q in range(5):
+            P[q, 3:A + 3, 3:Bn + 3] = cs[q]
+        for b in range(Bn):
+            win = P[:, 0:7, b:b + 7]
+            for fc in range(2):
+                f, i, j, sg = LB.side_cell(side, fc, b, X, X)
+                ghost = (f == 0 and i == X - 1) or (f == 1 and j == X - 1)
+                if ghost or LB.ring_side(f, i, j, X, X) != side:
+                    continue
+                delta = sg * np.sum(K[fc] * win)
+                exp.append(2 * V[f, i, j] - Vm[f, i, j] + delta)
+                got.append(Vn[f, i, j])
+    exp, got = np.array(exp), np.array(got)
+    scale = np.abs(got).max()  # the ring's own magnitude
+    assert scale > 0
+    print(f"ring cells {len(exp)}, max |err| / max|V_ring| = {np.abs(exp - got).max() / scale:.2e}")
+    nring = 0
+    for i in range(X):
+        for j in range(X):
+            nring += (i != X - 1) and not (1 <= i < X - 2 and 1 <= j < X - 1)  # vx
+            nring += (j != X - 1) and not (1 <= i < X - 1 and 1 <= j < X - 2)  # vz
+    assert len(exp) == nring  # every ring cell, on exactly one side
+    assert np.abs(exp - got).max() <= 2e-5 * scale
+    # live pad cells, second-order update (depth 1 .. L-2: stencils stay in the pad)
+    vx, vz = V
+    dx = dz = c.dx
+    gxx = np.zeros_like(vx); gzz = np.zeros_like(vx); gxz = np.zeros_like(vx); gzx = np.zeros_like(vx)
+    gxx[1:-1, 1:-1] = (vx[1:-1, 1:-1] - vx[:-2, 1:-1]) / dx
+    gzz[1:-1, 1:-1] = (vz[1:-1, 1:-1] - vz[1:-1, :-2]) / dz
+    gxz[:-1, :-1] = (vx[:-1, 1:] - vx[:-1, :-1]) / dz
+    gzx[:-1, :-1] = (vz[1:, :-1] - vz[:-1, :-1]) / dx
+    th = p2 * (gxx + gzz)
+    om = q2 * (gxz - gzx)
+    ii, jj = np.meshgrid(np.arange(X), np.arange(X), indexing="ij")
+    dep = np.minimum(np.minimum(ii, X - 1 - ii), np.minimum(jj, X - 1 - jj))
+    sel = (dep >= 1) & (dep <= L - 2)
+    Rx = np.zeros_like(vx)
+    Rz = np.zeros_like(vx)
+    Rx[1:-2, 1:-1] = ((th[2:-1, 1:-1] - th[1:-2, 1:-1]) / dx + (om[1:-2, 1:-1] - om[1:-2, :-2]) / dz
+                      + 2 * ((q2[1:-2, 1:-1] * gzx[1:-2, 1:-1] - q2[1:-2, :-2] * gzx[1:-2, :-2]) / dz
+                             - (q2[2:-1, 1:-1] * gzz[2:-1, 1:-1] - q2[1:-2, 1:-1] * gzz[1:-2, 1:-1]) / dx))
+    Rz[1:-1, 1:-2] = ((th[1:-1, 2:-1] - th[1:-1, 1:-2]) / dz - (om[1:-1, 1:-2] - om[:-2, 1:-2]) / dx
+                      + 2 * ((q2[1:-1, 1:-2] * gxz[1:-1, 1:-2] - q2[:-2, 1:-2] * gxz[:-2, 1:-2]) / dx
+                             - (q2[1:-1, 2:-1] * gxx[1:-1, 2:-1] - q2[1:-1, 1:-2] * gxx[1:-1, 1:-2]) / dz))
+    live_x = np.zeros_like(sel); live_x[1:-2, 1:-1] = True
+    live_z = np.zeros_like(sel); live_z[1:-1, 1:-2] = True
+    ex = 2 * vx - Vm[0] + c.dt ** 2 * Rx
+    ez = 2 * vz - Vm[1] + c.dt ** 2 * Rz
+    mx, mz = sel & live_x, sel & live_z
+    err = max(np.abs(ex - Vn[0])[mx].max(), np.abs(ez - Vn[1])[mz].max())
+    scale = max(np.abs(Vn[0][mx]).max(), np.abs(Vn[1][mz]).max())
+    print(f"pad cells {mx.sum() + mz.sum()}, max |err| / max|V_pad| = {err / scale:.2e}")
+    assert err <= 2e-5 * scale
+
+
+def test_trained_network_beats_the_band_without_it(LB, weights):
+    """Reflected energy against the 3x extended run (dB, lower is better) on
+    cases the network never saw: the trained network against the same 8-cell
+    band without it (ring held at zero) and the reference's CPML profile on the
+    same 8 cells."""
+    rows = []
+    for k in range(3):
+        c = LB.Case(96, 9000 + k)
+        r = LB.reflection_db(c, weights, variants=("learned", "no_network", "cpml_band"))
+        rows.append(r)
+        print(f"case {k}: vp {c.vp_ext.min():.0f}-{c.vp_ext.max():.0f} m/s, fm {c.fm:.1f} Hz: "
+              + ", ".join(f"{a} {b:.1f} dB" for a, b in r.items()))
+    for r in rows:
+        assert r["learned"] < -12.0
+        assert r["learned"] < r["no_network"] - 8.0
+        assert r["learned"] < r["cpml_band"] - 8.0
+
+
+def test_divergence_falls_back_to_cpml(LB, weights):
+    from ouc_amd import abi
+    from ouc_amd.engine import Engine, OucError
+
+    c = LB.Case(48, 21, nt=300)
+    bad = 300.0 * weights
+    with Engine(c.small_params(abi.BOUNDARY_CPML, npml=40), c.vp, c.vs) as ref:
+        seis_ref = ref.forward_shot(c.shot())
+    with Engine(c.small_params(abi.BOUNDARY_LEARNED_FALLBACK), c.vp, c.vs) as e:
+        e.set_learned_boundary(bad)
+        with pytest.warns(RuntimeWarning):
+            seis = e.forward_shot(c.shot())
+        assert e.fell_back
+        assert e.timings()["fallback_shots"] == 1
+        np.testing.assert_array_equal(seis, seis_ref)
+        # a stable network does not fall back
+        e.set_learned_boundary(weights)
+        seis2 = e.forward_shot(c.shot())
+        assert not e.fell_back
+        assert np.isfinite(seis2).all()
+    with Engine(c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs) as e:
+        e.set_learned_boundary(bad)
+        with pytest.raises(OucError, match="diverged"):
+            e.forward_shot(c.shot())
+
+
+def test_learned_boundary_rtm_batch_runs_and_stacks(LB, weights):
+    """RTM with sources and receivers on the first interior row, next to the
+    learned band: the stack equals the sum of single-shot images, and the image
+    is close to the one with the full 40-cell CPML -- closer than with the same
+    band without the network."""
+    from ouc_amd import abi
+    from ouc_amd.engine import Engine
+
+    c = LB.Case(64, 31, nt=400)
+    shots = [dict(sx=16, sz=0, rec_z=0, fm=20.0), dict(sx=48, sz=0, rec_z=0, fm=20.0)]
+    with Engine(c.small_params(abi.BOUNDARY_CPML, npml=40), c.vp, c.vs) as e:
+        ref = e.rtm_batch(shots)["stack_pp"]
+    rels = {}
+    for name, w in (("learned", weights), ("no_network", None)):
+        with Engine(c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs) as e:
+            e.set_learned_boundary(w)
+            st = e.rtm_batch(shots)
+            if name == "learned":
+                singles = [e.rtm_shot(s)["lap_pp"] for s in shots]
+                np.testing.assert_allclose(st["stack_pp"], singles[0] + singles[1], rtol=1e-5,
+                                           atol=1e-6 * np.abs(singles[0]).max())
+        rels[name] = np.linalg.norm(st["stack_pp"] - ref) / np.linalg.norm(ref)
+    print("PP stack vs 40-cell CPML, rel L2: " + ", ".join(f"{k} {v:.3f}" for k, v in rels.items()))
+    assert rels["learned"] < 0.3
+    assert rels["learned"] < 0.5 * rels["no_network"]
+
+
+@pytest.mark.parametrize("boundary", ["cpml", "learned"])
+def test_reconstruction_reverses_the_forward_run(LB, weights, boundary):
+    """EXACT semantics: the source wavefield reconstructed backwards from the
+    boundary strips equals the forward run's V^t (time reversibility of
+    reshot_u / reshot_s, P-S-seperating.cu:890-1042), with the source on the
+    first interior row where the ring, the pad and the source meet."""
+    from ouc_amd import abi
+    from ouc_amd.engine import Engine
+
+    c = LB.Case(64, 31, nt=400)
+    shot = dict(sx=16, sz=0, rec_z=20, fm=20.0)
+    if boundary == "cpml":
+        p, L = c.small_params(abi.BOUNDARY_CPML, npml=40), 40
+    else:
+        p, L = c.small_params(abi.BOUNDARY_LEARNED), c.band
+    p.semantics = abi.SEM_EXACT
+    times = [390, 300, 200, 100, 50]
+    with Engine(p, c.vp, c.vs) as e:
+        if boundary == "learned":
+            e.set_learned_boundary(weights)
+        fw = e.forward_snapshots(shot, [t - 1 for t in times[::-1]], L, L, 64, 64)[::-1]
+        rc = e.recon_snapshots(shot, times, L, L, 64, 64)
+    errs = [float(np.linalg.norm(rc[k] - fw[k]) / np.linalg.norm(fw[k])) for k in range(len(times))]
+    print(f"{boundary}: reconstruction rel L2 at t={times}: " + " ".join(f"{x:.1e}" for x in errs))
+    assert errs[0] < 1e-6 and errs[-1] < 1e-4
