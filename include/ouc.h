@@ -170,6 +170,13 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
  * times itself (P-S-seperating.cu:1663). */
 ouc_status ouc_set_profiling(ouc_ctx *c, int32_t on);
 
+/* HIP graphs (default on): the fused engine captures each time loop of a
+ * batch (forward, then receiver + reconstruction) with stream capture the
+ * first time a batch size is run and replays the graph on later calls; eager
+ * launches are used while profiling is on.  No counterpart in the reference,
+ * which launches every kernel from the host loop (P-S-seperating.cu:2258-2364). */
+ouc_status ouc_set_graphs(ouc_ctx *c, int32_t on);
+
 /* Timing of the last forward/RTM/batch call on this context. */
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);
 

@@ -597,6 +597,14 @@ ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_
     return OUC_OK;
 }
 
+ouc_status ouc_set_graphs(ouc_ctx *c, int32_t on) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    return guarded(c, [&] {
+        c->graphs = on != 0;
+        if (!c->graphs) fused_drop_graphs(c);
+    });
+}
+
 ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {

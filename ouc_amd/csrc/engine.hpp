@@ -63,6 +63,7 @@ struct ouc_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool profiling = false;
+    bool graphs = true; // replay captured time loops (fused engine, profiling off)
 
     std::vector<float> vp_pad; // host copy (mute velocity)
     std::vector<float> vp_in, vs_in; // interior model (the CPML fallback context is built from it)

@@ -57,7 +57,50 @@ struct FusedState {
     int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
     bool lb_have = false;  // composed filter present (else the ring is held at zero)
     DevBuf rec6;           // [6][nt][nx] forward seismograms of one shot
+    // time loops captured as HIP graphs, replayed while the key matches
+    struct LoopGraph {
+        int B = -1, rtm = -1, ncomp = -1;
+        hipGraphExec_t exec = nullptr;
+        void drop() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+            B = -1;
+        }
+        bool matches(int b, int r, int n) const { return exec && B == b && rtm == r && ncomp == n; }
+    } gfwd, gbwd;
+    void drop_graphs() { gfwd.drop(); gbwd.drop(); }
+    ~FusedState() { drop_graphs(); }
 };
+
+// Enqueues `body` on stream s directly, or replays the graph it was captured
+// into the first time for this key (HIP stream capture).
+template <class F>
+void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int rtm, int ncomp, F &&body) {
+    if (!graph) {
+        body();
+        return;
+    }
+    if (!lg.matches(B, rtm, ncomp)) {
+        lg.drop();
+        hipGraph_t gr = nullptr;
+        OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        OUC_HIP(hipStreamEndCapture(s, &gr));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        OUC_HIP(e);
+        lg.exec = ex;
+        lg.B = B; lg.rtm = rtm; lg.ncomp = ncomp;
+    }
+    OUC_HIP(hipGraphLaunch(lg.exec, s));
+}
 
 constexpr int kMonStride = 8; // steps between boundary-monitor launches
 
@@ -159,6 +202,7 @@ void ensure(ouc_ctx *c, int shots) {
     FusedState *st = fs(c);
     const int want16 = c->p.strip_precision == OUC_STRIP_FP16;
     if (st->cap >= shots && st->fp16 == want16) return;
+    st->drop_graphs();
     const Grid &g = c->g;
     st->g = make_fgrid(c);
     st->plane = st->g.plane;
@@ -399,7 +443,9 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     a.zr = zr;
     a.rec = ncomp == 6 ? st->rec6.f() : st->rec.f();
     a.ncomp = ncomp;
+    const bool graph = c->graphs && probe.ev.empty() && !snap && !rsnap;
     int rb = 0, ksnap = 0;
+    run_loop(s, graph, st->gfwd, B, rtm, ncomp, [&] {
     for (int t = 0; t < nt; ++t) {
         a.t = t;
         a.rb = rb;
@@ -420,6 +466,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         rb = 1 - rb;
     }
     OUC_HIP(hipGetLastError());
+    });
+    rb = nt & 1; // V^nt is in buffer nt mod 2 (also when the loop was replayed)
     if (!rtm) return;
     if (rec_in) {
         // observed records [2][nx][nt] (receiver-major) replace the modelled ones of shot 0
@@ -448,6 +496,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     r.inj = st->rec.f();
     r.npp = st->acc[0].f(); r.nps = st->acc[1].f(); r.den = st->acc[2].f();
     int rbr = 0;
+    run_loop(s, graph, st->gbwd, B, rtm, ncomp, [&] {
     for (int t = nt - 1; t >= 0; --t) {
         if (t < nt - 1) {
             // theta^{t+1} from V^{t+1}: ring totals from slot t+1 (FAITHFUL, one step late as the
@@ -493,6 +542,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         rbr = 1 - rbr;
     }
     OUC_HIP(hipGetLastError());
+    });
 }
 
 // Images of shot b of the batch into c->img[0..3].
@@ -812,6 +862,7 @@ float bf16_round(float x) {
 // the BF16 values and is rounded once to FP32.  weights == NULL: no network.
 void fused_set_learned(ouc_ctx *c, const float *weights, int n) {
     FusedState *st = fs(c);
+    st->drop_graphs(); // the ring kernel's filter argument changes
     if (!weights) {
         st->lb_have = false;
         return;
@@ -881,3 +932,5 @@ void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *
     OUC_HIP(hipMemcpyAsync(out, buf.ptr, buf.bytes, hipMemcpyDeviceToHost, c->stream));
     OUC_HIP(hipStreamSynchronize(c->stream));
 }
+
+void fused_drop_graphs(ouc_ctx *c) { fs(c)->drop_graphs(); }

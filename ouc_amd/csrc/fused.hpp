@@ -26,5 +26,7 @@ void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *
 // Learned-boundary network weights (NULL: the pad without the network).
 void fused_set_learned(ouc_ctx *c, const float *weights, int n);
 void fused_get_learned_filter(ouc_ctx *c, float *taps);
+// Forgets the captured time-loop graphs.
+void fused_drop_graphs(ouc_ctx *c);
 // The CPML context a learned-boundary context falls back to (engine.hip).
 ouc_ctx *fallback_ctx(ouc_ctx *c);

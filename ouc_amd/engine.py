@@ -49,6 +49,8 @@ def lib():
     L.ouc_write_image_files.restype = C.c_int
     L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, P(C.c_uint16), P(C.c_int8)]
     L.ouc_strip_codec_fp16.restype = C.c_int
+    L.ouc_set_graphs.argtypes = [C.c_void_p, C.c_int32]
+    L.ouc_set_graphs.restype = C.c_int
     L.ouc_set_learned_boundary.argtypes = [C.c_void_p, fp, C.c_int32]
     L.ouc_set_learned_boundary.restype = C.c_int
     L.ouc_get_learned_filter.argtypes = [C.c_void_p, fp]
@@ -163,6 +165,10 @@ class Engine:
         if per_shot:
             out["per_shot_ms"] = ms
         return out
+
+    def set_graphs(self, on=True):
+        """Replay captured time-loop graphs (fused engine; default on)."""
+        self._check(lib().ouc_set_graphs(self._h, int(bool(on))))
 
     def set_learned_boundary(self, weights):
         """FP32 master weights of the boundary network (3312 values), or None

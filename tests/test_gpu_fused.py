@@ -61,3 +61,54 @@ def test_fused_batch_is_sum_of_single_shots(oracle):
     print(f"[batch] max |stack - sum| pp {np.nanmax(np.abs(out['stack_pp'] - spp)):.3e}", flush=True)
     assert np.array_equal(out["stack_pp"], spp, equal_nan=True)
     assert np.array_equal(out["stack_ps"], sps, equal_nan=True)
+
+
+def test_fused_forward_seismograms_vs_oracle(oracle):
+    """The fused engine's forward path (ouc_forward_shot with ENGINE_FUSED):
+    the six seismogram components against the FP64 oracle."""
+    p, vp, vs = _case(nt=700)
+    shot = dict(sx=80, sz=5, rec_z=5, fm=25.0)
+    with Engine(p, vp, vs) as e:
+        seis = e.forward_shot(shot)
+    ref = oracle.forward_shot(p, vp, vs, shot, 64)
+    names = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
+    errs = {n: rel_l2(seis[k], ref[k]) for k, n in enumerate(names)}
+    print("[fused forward] rel-L2 vs FP64:", " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    for n in ("vx", "vz", "vpx", "vpz"):
+        assert errs[n] < 1e-5, (n, errs[n])
+    for n in ("vsx", "vsz"):
+        assert errs[n] < 1e-3, (n, errs[n])
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_graph_replays_equal_eager_launches(learned):
+    """Time loops captured as HIP graphs and replayed give, bit for bit, what
+    eager launches give -- on the first replay and on later ones, with eager
+    work of another batch size in between."""
+    if learned:
+        from ouc_amd import learned as LB
+
+        c = LB.Case(64, 31, nt=300)
+        p, vp, vs = c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs
+        w = LB.load_weights()
+    else:
+        p, vp, vs = _case(nt=300)
+    shots = [dict(sx=s, sz=0, rec_z=0, fm=22.0, mute=1) for s in (10, 30, 50)]
+    with Engine(p, vp, vs) as e:
+        if learned:
+            e.set_learned_boundary(w)
+        e.set_graphs(False)
+        ref = e.rtm_batch(shots)
+        fref = e.forward_shot(shots[0])
+        e.set_graphs(True)
+        outs = []
+        for k in range(3):
+            outs.append(e.rtm_batch(shots))
+            f = e.forward_shot(shots[0])      # another captured loop (forward only, one shot)
+            np.testing.assert_array_equal(f, fref)
+            e.set_graphs(False)
+            e.rtm_batch(shots[:2])            # eager work between replays
+            e.set_graphs(True)
+    for o in outs:
+        np.testing.assert_array_equal(o["stack_pp"], ref["stack_pp"])
+        np.testing.assert_array_equal(o["stack_ps"], ref["stack_ps"])
