@@ -109,6 +109,9 @@ typedef struct ouc_shot {
     float fm;                /* Ricker peak frequency, Hz; also sets the mute's t0 */
     const float *wavelet;    /* optional nt samples; NULL: Ricker(fm) */
     int32_t mute;            /* 1: direct-wave mute of the records (P-S-seperating.cu:1634-1657) */
+    int32_t nt;              /* steps of this shot, 0 = the context's nt.  Fused RTM only: in a batch
+                                a shorter shot starts late (zero wavelet before its start) and images
+                                only its own steps, so its image equals a run of nt steps alone */
 } ouc_shot;
 
 /* Per-call timing, measured with HIP events on the stream the kernels run on. */
@@ -176,6 +179,33 @@ ouc_status ouc_set_profiling(ouc_ctx *c, int32_t on);
  * launches are used while profiling is on.  No counterpart in the reference,
  * which launches every kernel from the host loop (P-S-seperating.cu:2258-2364). */
 ouc_status ouc_set_graphs(ouc_ctx *c, int32_t on);
+
+/* Shots per fused launch: ouc_rtm_batch runs at most `shots` shots together
+ * (0: as many as 85% of free device memory holds), so contexts that share a
+ * GPU can split its memory.  bytes_per_shot (optional) returns what one shot
+ * of a batch holds on the device. */
+ouc_status ouc_set_batch_capacity(ouc_ctx *c, int32_t shots, double *bytes_per_shot);
+
+/* Tile of the fused engine's interior kernel: x columns each 256-lane block
+ * streams and the columns of loads kept in flight (1..3).  Results do not
+ * depend on the tile (each cell's arithmetic is the same); speed does.  No
+ * counterpart in the reference, whose launch shape is fixed
+ * (P-S-seperating.cu:1978-1979). */
+typedef struct ouc_tile {
+    int32_t core_cols;
+    int32_t prefetch;
+} ouc_tile;
+ouc_status ouc_set_tile(ouc_ctx *c, const ouc_tile *t);
+ouc_status ouc_get_tile(ouc_ctx *c, ouc_tile *t);
+
+/* Autotuner: times the interior kernel on this context's grid with nshots
+ * shots, `reps` launches per candidate (core_cols {32, 64, 128, 256} x
+ * prefetch {1, 2, 3}: 12 candidates, ms_per_launch[12] optional), selects the
+ * fastest and returns it in *best. */
+ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best, float *ms_per_launch);
+
+/* gfx architecture name of a HIP device (e.g. "gfx950"), for tuning caches. */
+ouc_status ouc_device_arch(int32_t device, char *buf, int32_t len);
 
 /* Timing of the last forward/RTM/batch call on this context. */
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);

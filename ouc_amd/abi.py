@@ -50,7 +50,7 @@ class Model(C.Structure):
 class Shot(C.Structure):
     _fields_ = [
         ("sx", C.c_int32), ("sz", C.c_int32), ("rec_z", C.c_int32),
-        ("fm", C.c_float), ("wavelet", C.POINTER(C.c_float)), ("mute", C.c_int32),
+        ("fm", C.c_float), ("wavelet", C.POINTER(C.c_float)), ("mute", C.c_int32), ("nt", C.c_int32),
     ]
 
 
@@ -85,3 +85,7 @@ def dptr(a):
         return C.POINTER(C.c_double)()
     assert a.dtype.name == "float64" and a.flags["C_CONTIGUOUS"], "need C-contiguous float64"
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Tile(C.Structure):
+    _fields_ = [("core_cols", C.c_int32), ("prefetch", C.c_int32)]

@@ -163,6 +163,13 @@ void check_shot(const ouc_ctx *c, const ouc_shot *s) {
         throw Error(OUC_INVALID_ARG, "source outside the interior");
     if (s->rec_z < 0 || s->rec_z >= c->g.nz) throw Error(OUC_INVALID_ARG, "receiver row outside the interior");
     if (!s->wavelet && !(s->fm > 0)) throw Error(OUC_INVALID_ARG, "fm must be positive when no wavelet is given");
+    if (s->nt < 0 || s->nt > c->g.nt || (s->nt > 0 && s->nt < 2)) throw Error(OUC_INVALID_ARG, "shot nt out of range");
+}
+
+// Paths that run exactly the context's nt steps.
+void check_full_length(const ouc_ctx *c, const ouc_shot *s) {
+    if (s->nt != 0 && s->nt != c->g.nt)
+        throw Error(OUC_UNSUPPORTED, "a shot nt other than the context's is supported by fused RTM only");
 }
 
 std::vector<float> shot_wavelet(const ouc_ctx *c, const ouc_shot *s) {
@@ -423,6 +430,7 @@ ouc_status ouc_forward_shot(ouc_ctx *c, const ouc_shot *s, float *seis) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        check_full_length(c, s);
         if (c->p.engine == OUC_ENGINE_FUSED) {
             fused_forward_one(c, s, seis);
             return;
@@ -464,9 +472,11 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     return guarded(c, [&] {
         check_shot(c, s);
         if (c->p.engine == OUC_ENGINE_FUSED) {
+            if (rec_in) check_full_length(c, s);
             fused_rtm_one(c, s, rec_in, img_pp, img_ps, img_pp_lap, img_ps_lap);
             return;
         }
+        check_full_length(c, s);
         const Grid &g = c->g;
         std::vector<float> wav = shot_wavelet(c, s);
         c->ensure_strips();
@@ -605,6 +615,40 @@ ouc_status ouc_set_graphs(ouc_ctx *c, int32_t on) {
     });
 }
 
+ouc_status ouc_set_batch_capacity(ouc_ctx *c, int32_t shots, double *bytes_per_shot) {
+    if (!c || shots < 0) return fail(c, OUC_INVALID_ARG, "bad arguments");
+    return guarded(c, [&] {
+        c->batch_cap = shots;
+        if (bytes_per_shot) *bytes_per_shot = fused_shot_bytes(c);
+    });
+}
+
+ouc_status ouc_set_tile(ouc_ctx *c, const ouc_tile *t) {
+    if (!c || !t) return fail(c, OUC_INVALID_ARG, "bad arguments");
+    return guarded(c, [&] { fused_set_tile(c, t->core_cols, t->prefetch); });
+}
+
+ouc_status ouc_get_tile(ouc_ctx *c, ouc_tile *t) {
+    if (!c || !t) return fail(c, OUC_INVALID_ARG, "bad arguments");
+    return guarded(c, [&] { fused_get_tile(c, &t->core_cols, &t->prefetch); });
+}
+
+ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best, float *ms_per_launch) {
+    if (!c || !best) return fail(c, OUC_INVALID_ARG, "bad arguments");
+    return guarded(c, [&] {
+        if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "the autotuner tunes the fused engine");
+        fused_autotune(c, nshots, reps, &best->core_cols, &best->prefetch, ms_per_launch);
+    });
+}
+
+ouc_status ouc_device_arch(int32_t device, char *buf, int32_t len) {
+    if (!buf || len < 1) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(nullptr, OUC_DEVICE, "no such HIP device");
+    std::snprintf(buf, len, "%s", prop.gcnArchName);
+    return OUC_OK;
+}
+
 ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
@@ -623,6 +667,7 @@ ouc_status ouc_forward_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        check_full_length(c, s);
         if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "snapshots need the fused engine");
         fused_snapshots(c, s, n, steps, x0, z0, w, h, out);
     });
@@ -633,6 +678,7 @@ ouc_status ouc_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const i
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        check_full_length(c, s);
         if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "snapshots need the fused engine");
         fused_recon_snapshots(c, s, n, steps, x0, z0, w, h, out);
     });

@@ -64,6 +64,7 @@ struct ouc_ctx {
     std::string err;
     bool profiling = false;
     bool graphs = true; // replay captured time loops (fused engine, profiling off)
+    int batch_cap = 0;  // shots per fused launch (0: as many as 85% of free HBM holds)
 
     std::vector<float> vp_pad; // host copy (mute velocity)
     std::vector<float> vp_in, vs_in; // interior model (the CPML fallback context is built from it)

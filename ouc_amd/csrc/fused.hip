@@ -34,6 +34,21 @@ namespace {
     default: { constexpr int NN = 8; __VA_ARGS__; } break;                       \
     }
 
+// k_core with the prefetch depth chosen at run time (1..3 for N = 5, 6; the
+// default elsewhere, to bound the number of instantiations).
+template <int N, int RECON>
+void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2, const float *q2,
+                 const FStep &a, const FRecon &rc) {
+    if constexpr (N == 5 || N == 6) {
+        switch (pf) {
+        case 1: k_core<N, RECON, 1><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
+        case 3: k_core<N, RECON, 3><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
+        default: break;
+        }
+    }
+    k_core<N, RECON, kCorePF><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc);
+}
+
 constexpr int kWaveArrays = 28; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O
 
 struct FusedState {
@@ -48,7 +63,7 @@ struct FusedState {
     long shot_strip = 0, shot_chunks = 0, nchunks = 0;
     int nslots = 0, fp16 = 0;
     DevBuf rec;            // [cap][2][nt][nx]
-    DevBuf srcpos, wav, counts;
+    DevBuf srcpos, wav, counts, t0;
     DevBuf tiles_s, tiles_v, tiles_rs, tiles_rv;
     int ntiles_s = 0, ntiles_v = 0, ntiles_rs = 0, ntiles_rv = 0;
     DevBuf img;            // [4][nx*nz] scratch for one shot's images
@@ -56,6 +71,7 @@ struct FusedState {
     DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
     int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
     bool lb_have = false;  // composed filter present (else the ring is held at zero)
+    int csx = kCoreSX, pf = kCorePF; // core tile
     DevBuf rec6;           // [6][nt][nx] forward seismograms of one shot
     // time loops captured as HIP graphs, replayed while the key matches
     struct LoopGraph {
@@ -128,6 +144,7 @@ FGrid make_fgrid(const ouc_ctx *c) {
     f.cx0 = g.L + g.N; f.cx1 = g.Xn - g.L - g.N;
     f.cz0 = g.L + g.N; f.cz1 = g.Zn - g.L - g.N;
     if (f.cx1 <= f.cx0 || f.cz1 <= f.cz0) { f.cx1 = f.cx0; f.cz1 = f.cz0; } // no core
+    f.csx = kCoreSX;
     f.lpad = g.learned ? g.L : 0;
     f.vlo = g.learned ? 0 : g.N;
     return f;
@@ -205,6 +222,7 @@ void ensure(ouc_ctx *c, int shots) {
     st->drop_graphs();
     const Grid &g = c->g;
     st->g = make_fgrid(c);
+    st->g.csx = st->csx;
     st->plane = st->g.plane;
     st->cap = shots;
     st->fp16 = want16;
@@ -227,6 +245,7 @@ void ensure(ouc_ctx *c, int shots) {
     }
     st->rec.alloc((size_t)shots * 2 * g.nt * g.nx * sizeof(float));
     st->srcpos.alloc((size_t)shots * 2 * sizeof(int));
+    st->t0.alloc((size_t)shots * sizeof(int));
     st->wav.alloc((size_t)shots * g.nt * sizeof(float));
     st->counts.alloc((size_t)shots * g.nx * sizeof(int32_t));
     st->img.alloc((size_t)4 * g.nx * g.nz * sizeof(float));
@@ -304,7 +323,7 @@ Launch launches(const FusedState *st, int B) {
     Launch L{};
     const int TZ = 256 - 4 * f.N;
     L.has_core = f.cx1 > f.cx0 && f.cz1 > f.cz0;
-    L.core = dim3((f.cx1 - f.cx0 + kCoreSX - 1) / kCoreSX, (f.cz1 - f.cz0 + TZ - 1) / TZ, B);
+    L.core = dim3((f.cx1 - f.cx0 + f.csx - 1) / f.csx, (f.cz1 - f.cz0 + TZ - 1) / TZ, B);
     L.band_s = dim3(st->ntiles_s, 1, B);
     L.band_v = dim3(st->ntiles_v, 1, B);
     L.recon_s = dim3(st->ntiles_rs, 1, B);
@@ -314,7 +333,7 @@ Launch launches(const FusedState *st, int B) {
     L.rg.cz0 = f.L + 2 * f.N; L.rg.cz1 = f.Zn - f.L - 2 * f.N;
     L.has_recon_core = L.rg.cx1 > L.rg.cx0 && L.rg.cz1 > L.rg.cz0;
     if (!L.has_recon_core) { L.rg.cx1 = L.rg.cx0; L.rg.cz1 = L.rg.cz0; }
-    L.recon_core = dim3((L.rg.cx1 - L.rg.cx0 + kCoreSX - 1) / kCoreSX, (L.rg.cz1 - L.rg.cz0 + TZ - 1) / TZ, B);
+    L.recon_core = dim3((L.rg.cx1 - L.rg.cx0 + f.csx - 1) / f.csx, (L.rg.cz1 - L.rg.cz0 + TZ - 1) / TZ, B);
     L.full = dim3((f.Zn + 63) / 64, (f.Xn + 3) / 4, B);
     return L;
 }
@@ -358,7 +377,7 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
     if (L.has_core) {
         bool p = pr.want(nlaunch++);
         if (p) OUC_HIP(hipEventRecord(pr.ev[2 * pr.used], s));
-        OUC_FDISPATCH(f.N, k_core<NN, 0><<<L.core, 256, 0, s>>>(f, w, c->p2.f(), c->q2.f(), a, FRecon{}));
+        OUC_FDISPATCH(f.N, launch_core<NN, 0>(st->pf, L.core, s, f, w, c->p2.f(), c->q2.f(), a, FRecon{}));
         if (p) { OUC_HIP(hipEventRecord(pr.ev[2 * pr.used + 1], s)); ++pr.used; }
     }
     OUC_FDISPATCH(f.N, k_band_velocity<NN><<<L.band_v, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_v.as<int2>(), a));
@@ -404,26 +423,29 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     const bool exact = c->p.semantics == OUC_SEM_EXACT;
     const int nt = g.nt;
     // per-shot inputs
-    std::vector<int> pos(2 * B);
-    std::vector<float> wavs((size_t)B * nt);
+    std::vector<int> pos(2 * B), start(B, 0);
+    std::vector<float> wavs((size_t)B * nt, 0.f);
     std::vector<int32_t> counts((size_t)B * g.nx, 0);
     int zr = -1;
     for (int b = 0; b < B; ++b) {
         const ouc_shot &sh = shots[b];
+        const int nts = sh.nt > 0 ? sh.nt : nt;
+        start[b] = nt - nts; // a shorter shot starts late and ends with the others
         pos[2 * b] = sh.sx + g.L;
         pos[2 * b + 1] = sh.sz + g.L;
         std::vector<float> w;
-        if (sh.wavelet) w.assign(sh.wavelet, sh.wavelet + nt);
-        else ricker(nt, sh.fm, g.dt, w);
-        std::copy(w.begin(), w.end(), wavs.begin() + (size_t)b * nt);
+        if (sh.wavelet) w.assign(sh.wavelet, sh.wavelet + nts);
+        else ricker(nts, sh.fm, g.dt, w);
+        std::copy(w.begin(), w.end(), wavs.begin() + (size_t)b * nt + start[b]);
         if (zr < 0) zr = sh.rec_z + g.L;
         if (sh.rec_z + g.L != zr) throw Error(OUC_INVALID_ARG, "all shots of a batch must share the receiver row");
         if (sh.mute) {
             std::vector<int32_t> cnt;
             mute_counts(g, sh, c->vp_pad[(size_t)(sh.sx + g.L) * g.P + zr], cnt);
-            std::copy(cnt.begin(), cnt.end(), counts.begin() + (size_t)b * g.nx);
+            for (int ix = 0; ix < g.nx; ++ix) counts[(size_t)b * g.nx + ix] = std::min(nt, cnt[ix] + start[b]);
         }
     }
+    OUC_HIP(hipMemcpyAsync(st->t0.ptr, start.data(), B * sizeof(int), hipMemcpyHostToDevice, s));
     OUC_HIP(hipMemcpyAsync(st->srcpos.ptr, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, s));
     OUC_HIP(hipMemcpyAsync(st->wav.ptr, wavs.data(), wavs.size() * sizeof(float), hipMemcpyHostToDevice, s));
     OUC_HIP(hipMemcpyAsync(st->counts.ptr, counts.data(), counts.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -495,6 +517,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     r.zr = zr;
     r.inj = st->rec.f();
     r.npp = st->acc[0].f(); r.nps = st->acc[1].f(); r.den = st->acc[2].f();
+    r.t0 = st->t0.as<int>();
     int rbr = 0;
     run_loop(s, graph, st->gbwd, B, rtm, ncomp, [&] {
     for (int t = nt - 1; t >= 0; --t) {
@@ -520,7 +543,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             OUC_FDISPATCH(f.N, k_recon_stress_f<NN><<<L.recon_s, dim3(64, 4), 0, s>>>(
                                    f, st->S, c->p2.f(), c->q2.f(), rc, ra, st->tiles_rs.as<int2>()));
             if (L.has_recon_core)
-                OUC_FDISPATCH(f.N, k_core<NN, 1><<<L.recon_core, 256, 0, s>>>(L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
+                OUC_FDISPATCH(f.N, launch_core<NN, 1>(st->pf, L.recon_core, s, L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
             OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, s>>>(f, st->S, c->q2.f(), rc, ra,
                                                                                         st->tiles_rv.as<int2>()));
             rbs = 1 - rbs;
@@ -572,6 +595,7 @@ int default_capacity(ouc_ctx *c, int nshots) {
     size_t avail = (size_t)((freeb + held) * 0.85);
     size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
     int cap = (int)std::max<size_t>(1, avail / per);
+    if (c->batch_cap > 0) cap = std::min(cap, c->batch_cap);
     return std::min(cap, nshots);
 }
 
@@ -934,3 +958,64 @@ void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *
 }
 
 void fused_drop_graphs(ouc_ctx *c) { fs(c)->drop_graphs(); }
+
+// Core tile (columns per block, prefetch depth).
+void fused_set_tile(ouc_ctx *c, int cols, int pf) {
+    if (cols < 8 || cols > 1024 || pf < 1 || pf > 3) throw Error(OUC_INVALID_ARG, "tile out of range");
+    FusedState *st = fs(c);
+    st->csx = cols;
+    st->pf = pf;
+    st->g.csx = cols;
+    st->drop_graphs();
+}
+
+void fused_get_tile(ouc_ctx *c, int *cols, int *pf) {
+    *cols = fs(c)->csx;
+    *pf = fs(c)->pf;
+}
+
+// Times k_core (forward mode) over `reps` launches for each candidate tile on
+// this grid with `nshots` shots; keeps the fastest.  ms[k] gets candidate k's
+// mean launch time (candidates: cols {32, 64, 128, 256} x pf {1, 2, 3}).
+void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_pf, float *ms) {
+    if (nshots < 1 || reps < 1) throw Error(OUC_INVALID_ARG, "nshots and reps must be positive");
+    ensure(c, nshots);
+    FusedState *st = fs(c);
+    hipStream_t s = c->stream;
+    OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, st->mem_s.bytes, s));
+    const int cols[4] = {32, 64, 128, 256}, pfs[3] = {1, 2, 3};
+    hipEvent_t e0, e1;
+    OUC_HIP(hipEventCreate(&e0));
+    OUC_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bc = st->csx, bp = st->pf, k = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 3; ++b, ++k) {
+            FGrid f = st->g;
+            f.csx = cols[a];
+            const int TZ = 256 - 4 * f.N;
+            dim3 grid((f.cx1 - f.cx0 + f.csx - 1) / f.csx, (f.cz1 - f.cz0 + TZ - 1) / TZ, nshots);
+            FStep st0{};
+            if (f.cx1 <= f.cx0) { if (ms) ms[k] = 0; continue; }
+            OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->p2.f(), c->q2.f(), st0, FRecon{})); // warm
+            OUC_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) {
+                st0.rb = r & 1;
+                OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->p2.f(), c->q2.f(), st0, FRecon{}));
+            }
+            OUC_HIP(hipEventRecord(e1, s));
+            OUC_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            OUC_HIP(hipEventElapsedTime(&t, e0, e1));
+            t /= reps;
+            if (ms) ms[k] = t;
+            if (t < best) { best = t; bc = cols[a]; bp = pfs[b]; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fused_set_tile(c, bc, bp);
+    *best_cols = bc;
+    *best_pf = bp;
+}
+
+double fused_shot_bytes(const ouc_ctx *c) { return (double)per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16); }

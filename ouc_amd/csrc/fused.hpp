@@ -26,6 +26,12 @@ void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *
 // Learned-boundary network weights (NULL: the pad without the network).
 void fused_set_learned(ouc_ctx *c, const float *weights, int n);
 void fused_get_learned_filter(ouc_ctx *c, float *taps);
+// Device bytes one shot of a fused batch holds (state, strips, records).
+double fused_shot_bytes(const ouc_ctx *c);
+// Core tile and its autotuner.
+void fused_set_tile(ouc_ctx *c, int cols, int pf);
+void fused_get_tile(ouc_ctx *c, int *cols, int *pf);
+void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_pf, float *ms);
 // Forgets the captured time-loop graphs.
 void fused_drop_graphs(ouc_ctx *c);
 // The CPML context a learned-boundary context falls back to (engine.hip).

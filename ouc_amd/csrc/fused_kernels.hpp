@@ -36,6 +36,7 @@ struct FGrid {
     float idx, idz, dt, dt2;
     float c[8];
     int cx0, cx1, cz0, cz1; // core
+    int csx;        // core tile: columns each k_core block streams
     int vlo;        // V is nonzero on [vlo, Xn-vlo) x [vlo, Zn-vlo): N with CPML, 0 with the learned pad
     int lpad;       // learned boundary: width of the low-order pad (0: CPML)
 };
@@ -68,6 +69,7 @@ struct FStep {
     const float *inj;        // RCV: [shot][2][nt][nx] records to inject, or NULL
     const float *spx, *spz;  // RCV: source P velocity level [shot][plane], or NULL
     float *npp, *nps, *den;  // RCV: imaging accumulators [shot][plane]
+    const int *t0;           // RCV: [shot] first step of each shot (later starts image only their own steps)
 };
 
 __device__ __forceinline__ bool in_update(const FGrid &g, int i, int j) {
@@ -118,6 +120,7 @@ __device__ __forceinline__ void record(const FGrid &g, const FStep &a, int i, in
 __device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k, int i, int j, float vx, float vz,
                                           float sx, float sz) {
     if (!a.spx || i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L) return;
+    if (a.t0 && a.t < a.t0[blockIdx.z]) return;
     float sa = a.spx[k], sb = a.spz[k];
     float rpx = vx - sx, rpz = vz - sz;
     a.npp[k] += sa * rpx + sb * rpz;
@@ -179,26 +182,25 @@ __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const 
     vz = w.V[rb][1][k];
 }
 
-constexpr int kCoreSX = 64;
-constexpr int kCorePF = 2; // columns of loads in flight ahead of use
+constexpr int kCoreSX = 64; // default columns per block
+constexpr int kCorePF = 2;  // default columns of loads in flight ahead of use
 
 // RECON = 1 runs the source reconstruction (reshot_u + reshot_s,
 // P-S-seperating.cu:890-1042) through the same pipeline: ring totals come from
 // the strips, the step is the reverse one, and the new source P velocity level
 // is written for the imaging condition.  Its core is [L+2N, Xn-L-2N)^2, so every
 // stress cell it forms lies off the ring.
-template <int N, int RECON>
+template <int N, int RECON, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float *__restrict__ p2,
                                               const float *__restrict__ q2, FStep a, FRecon rc) {
     constexpr int TZ = 256 - 4 * N;
     constexpr int Q = 2 * N + 1;
-    constexpr int PF = kCorePF;
     __shared__ float lv[2][256];
     __shared__ float ls[5][256];
     const int l = threadIdx.x;
     const int z = g.cz0 + blockIdx.y * TZ - 2 * N + l;
-    const int xs = g.cx0 + blockIdx.x * kCoreSX;
-    const int xe = min(xs + kCoreSX, g.cx1);
+    const int xs = g.cx0 + blockIdx.x * g.csx;
+    const int xe = min(xs + g.csx, g.cx1);
     const long so = (long)blockIdx.z * g.plane;
     const float *c = g.c;
     const int wb = 1 - a.rb;

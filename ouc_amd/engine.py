@@ -49,6 +49,16 @@ def lib():
     L.ouc_write_image_files.restype = C.c_int
     L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, P(C.c_uint16), P(C.c_int8)]
     L.ouc_strip_codec_fp16.restype = C.c_int
+    L.ouc_set_batch_capacity.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+    L.ouc_set_batch_capacity.restype = C.c_int
+    L.ouc_set_tile.argtypes = [C.c_void_p, P(abi.Tile)]
+    L.ouc_set_tile.restype = C.c_int
+    L.ouc_get_tile.argtypes = [C.c_void_p, P(abi.Tile)]
+    L.ouc_get_tile.restype = C.c_int
+    L.ouc_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(abi.Tile), fp]
+    L.ouc_autotune.restype = C.c_int
+    L.ouc_device_arch.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    L.ouc_device_arch.restype = C.c_int
     L.ouc_set_graphs.argtypes = [C.c_void_p, C.c_int32]
     L.ouc_set_graphs.restype = C.c_int
     L.ouc_set_learned_boundary.argtypes = [C.c_void_p, fp, C.c_int32]
@@ -77,12 +87,13 @@ def lib():
 def _shot_struct(shot, nt):
     """dict(sx, sz, rec_z, fm, wavelet=None, mute=0) -> (abi.Shot, keepalive)."""
     w = shot.get("wavelet")
+    nts = int(shot.get("nt", 0))
     if w is not None:
         w = np.ascontiguousarray(w, np.float32)
-        if w.size < nt:
-            raise ValueError("wavelet shorter than nt")
+        if w.size < (nts or nt):
+            raise ValueError("wavelet shorter than the shot's steps")
     s = abi.Shot(int(shot["sx"]), int(shot["sz"]), int(shot["rec_z"]), float(shot.get("fm", 0.0)), abi.fptr(w),
-                 int(shot.get("mute", 0)))
+                 int(shot.get("mute", 0)), nts)
     return s, w
 
 
@@ -165,6 +176,31 @@ class Engine:
         if per_shot:
             out["per_shot_ms"] = ms
         return out
+
+    def set_batch_capacity(self, shots):
+        """At most `shots` shots per fused launch (0: by free memory); returns
+        the device bytes one shot of a batch holds."""
+        b = C.c_double()
+        self._check(lib().ouc_set_batch_capacity(self._h, int(shots), C.byref(b)))
+        return b.value
+
+    def set_tile(self, core_cols, prefetch):
+        t = abi.Tile(int(core_cols), int(prefetch))
+        self._check(lib().ouc_set_tile(self._h, C.byref(t)))
+
+    def tile(self):
+        t = abi.Tile()
+        self._check(lib().ouc_get_tile(self._h, C.byref(t)))
+        return int(t.core_cols), int(t.prefetch)
+
+    def autotune(self, nshots, reps=5):
+        """Times the 12 candidate tiles on this grid, keeps the fastest;
+        returns ((core_cols, prefetch), {(cols, pf): ms per launch})."""
+        t = abi.Tile()
+        ms = np.zeros(12, np.float32)
+        self._check(lib().ouc_autotune(self._h, int(nshots), int(reps), C.byref(t), abi.fptr(ms)))
+        cands = [(c, p) for c in (32, 64, 128, 256) for p in (1, 2, 3)]
+        return (int(t.core_cols), int(t.prefetch)), {k: float(v) for k, v in zip(cands, ms)}
 
     def set_graphs(self, on=True):
         """Replay captured time-loop graphs (fused engine; default on)."""
@@ -275,3 +311,11 @@ def lb_train(x, y, mask, weights, epochs, lr, device=0):
     _lb_check(lib().ouc_lb_train(int(device), M, H, W, abi.fptr(x), abi.fptr(y), abi.fptr(mk), int(epochs),
                                  float(lr), abi.fptr(w), abi.fptr(hist)))
     return w, hist
+
+
+def device_arch(device=0):
+    buf = C.create_string_buffer(64)
+    st = lib().ouc_device_arch(int(device), buf, 64)
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
+    return buf.value.decode()
