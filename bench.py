@@ -44,6 +44,7 @@ def parse():
                     help="uneven serving batch instead: 32 shots on grids 512x256..4096x2048, nt 2000-8000, "
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
     ap.add_argument("--serve-shots", type=int, default=32)
+    ap.add_argument("--serve-schedule", choices=["sjf", "concurrent"], default="sjf")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -194,7 +195,7 @@ def serve_bench(a):
 
     jobs = serve.make_jobs(n=a.serve_shots)
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
-    lat, wall, groups = serve.serve(jobs, log=log)
+    lat, wall, groups = serve.serve(jobs, schedule=a.serve_schedule, log=log)
     s = serve.summary(jobs, lat, wall, groups)
     line = {
         "metric": "uneven serving batch: RTM shots/hour and per-shot latency p50/p95",
@@ -203,7 +204,7 @@ def serve_bench(a):
         "dtype": "f32", "data": "synthetic (seeded layered models per grid, random shots)",
         "config": {"workload": f"{len(jobs)} RTM shots, grids 512x256/1024x512/2048x1024/4096x2048, nt 2000-8000, "
                                "learned 8-cell boundary with CPML fallback, FP16 strips, autotuned tiles, "
-                               "groups on concurrent streams"},
+                               f"per-grid batches, schedule {a.serve_schedule}"},
         "latency_ms_p50": s["latency_ms_p50"], "latency_ms_p95": s["latency_ms_p95"], "groups": s["groups"],
     }
     print(json.dumps(line))

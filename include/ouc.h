@@ -199,9 +199,9 @@ ouc_status ouc_set_tile(ouc_ctx *c, const ouc_tile *t);
 ouc_status ouc_get_tile(ouc_ctx *c, ouc_tile *t);
 
 /* Autotuner: times the interior kernel on this context's grid with nshots
- * shots, `reps` launches per candidate (core_cols {32, 64, 128, 256} x
- * prefetch {1, 2, 3}: 12 candidates, ms_per_launch[12] optional), selects the
- * fastest and returns it in *best. */
+ * shots, `reps` launches per candidate (core_cols {8, 16, 32, 64, 128, 256}
+ * x prefetch {1, 2, 3}: 18 candidates, ms_per_launch[18] optional), selects
+ * the fastest and returns it in *best. */
 ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best, float *ms_per_launch);
 
 /* gfx architecture name of a HIP device (e.g. "gfx950"), for tuning caches. */
@@ -228,13 +228,19 @@ ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, uint1
 /* Learned absorbing boundary (boundary LEARNED / LEARNED_FALLBACK; no
  * counterpart in the reference, whose only boundary is the CPML of
  * P-S-seperating.cu:2067-2105).  The outer `band` cells run second-order
- * staggered differences; a three-layer 3x3 convolutional network (5 -> 16 ->
+ * staggered differences (with a weak sponge, see ouc_set_learned_damping); a three-layer 3x3 convolutional network (5 -> 16 ->
  * 16 -> 2 channels, linear, BF16 weights, FP32 accumulation) sets the outermost
  * ring from the local velocity increment and stress.  weights: FP32 masters,
  * n = 3312 (W1 [16][5][3][3], W2 [16][16][3][3], W3 [2][16][3][3]), rounded to
  * BF16 here; NULL holds the ring at zero (the pad without the network).
  * A context with the learned boundary starts without a network. */
 ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n);
+
+/* Weak sponge in the learned pad: the velocity increment of pad cells at depth
+ * d is scaled by 1 - beta ((band - d) / band)^2 each step (0 <= beta < 1;
+ * default 0.01).  It removes the slow growth the one-step-trained ring shows
+ * after several thousand steps, at about 1-2 dB of reflection. */
+ouc_status ouc_set_learned_damping(ouc_ctx *c, float beta);
 
 /* The composed 7x7 filter [2][5][7][7] the ring kernel applies (test hook). */
 ouc_status ouc_get_learned_filter(ouc_ctx *c, float *taps);

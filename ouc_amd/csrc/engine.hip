@@ -657,6 +657,14 @@ ouc_status ouc_set_learned_boundary(ouc_ctx *c, const float *weights, int32_t n)
     });
 }
 
+ouc_status ouc_set_learned_damping(ouc_ctx *c, float beta) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    return guarded(c, [&] {
+        if (!c->g.learned) throw Error(OUC_INVALID_ARG, "the context does not use the learned boundary");
+        fused_set_learned_damping(c, beta);
+    });
+}
+
 ouc_status ouc_get_learned_filter(ouc_ctx *c, float *taps) {
     if (!c || !taps) return fail(c, OUC_INVALID_ARG, "bad arguments");
     return guarded(c, [&] { fused_get_learned_filter(c, taps); });

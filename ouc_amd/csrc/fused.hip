@@ -72,6 +72,7 @@ struct FusedState {
     int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
     bool lb_have = false;  // composed filter present (else the ring is held at zero)
     int csx = kCoreSX, pf = kCorePF; // core tile
+    float lb_beta = 0.01f;           // learned pad damping (the default keeps 8000-step runs stable)
     DevBuf rec6;           // [6][nt][nx] forward seismograms of one shot
     // time loops captured as HIP graphs, replayed while the key matches
     struct LoopGraph {
@@ -223,6 +224,7 @@ void ensure(ouc_ctx *c, int shots) {
     const Grid &g = c->g;
     st->g = make_fgrid(c);
     st->g.csx = st->csx;
+    st->g.lb_beta = st->lb_beta;
     st->plane = st->g.plane;
     st->cap = shots;
     st->fp16 = want16;
@@ -976,20 +978,20 @@ void fused_get_tile(ouc_ctx *c, int *cols, int *pf) {
 
 // Times k_core (forward mode) over `reps` launches for each candidate tile on
 // this grid with `nshots` shots; keeps the fastest.  ms[k] gets candidate k's
-// mean launch time (candidates: cols {32, 64, 128, 256} x pf {1, 2, 3}).
+// mean launch time (candidates: cols {8, 16, 32, 64, 128, 256} x pf {1, 2, 3}).
 void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_pf, float *ms) {
     if (nshots < 1 || reps < 1) throw Error(OUC_INVALID_ARG, "nshots and reps must be positive");
     ensure(c, nshots);
     FusedState *st = fs(c);
     hipStream_t s = c->stream;
     OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, st->mem_s.bytes, s));
-    const int cols[4] = {32, 64, 128, 256}, pfs[3] = {1, 2, 3};
+    const int cols[6] = {8, 16, 32, 64, 128, 256}, pfs[3] = {1, 2, 3};
     hipEvent_t e0, e1;
     OUC_HIP(hipEventCreate(&e0));
     OUC_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int bc = st->csx, bp = st->pf, k = 0;
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 6; ++a)
         for (int b = 0; b < 3; ++b, ++k) {
             FGrid f = st->g;
             f.csx = cols[a];
@@ -1019,3 +1021,11 @@ void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_
 }
 
 double fused_shot_bytes(const ouc_ctx *c) { return (double)per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16); }
+
+void fused_set_learned_damping(ouc_ctx *c, float beta) {
+    if (!(beta >= 0.f && beta < 1.f)) throw Error(OUC_INVALID_ARG, "damping must be in [0, 1)");
+    FusedState *st = fs(c);
+    st->lb_beta = beta;
+    st->g.lb_beta = beta;
+    st->drop_graphs();
+}

@@ -26,6 +26,7 @@ void fused_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int n, const int32_t *
 // Learned-boundary network weights (NULL: the pad without the network).
 void fused_set_learned(ouc_ctx *c, const float *weights, int n);
 void fused_get_learned_filter(ouc_ctx *c, float *taps);
+void fused_set_learned_damping(ouc_ctx *c, float beta);
 // Device bytes one shot of a fused batch holds (state, strips, records).
 double fused_shot_bytes(const ouc_ctx *c);
 // Core tile and its autotuner.

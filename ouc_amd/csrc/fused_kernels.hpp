@@ -39,6 +39,7 @@ struct FGrid {
     int csx;        // core tile: columns each k_core block streams
     int vlo;        // V is nonzero on [vlo, Xn-vlo) x [vlo, Zn-vlo): N with CPML, 0 with the learned pad
     int lpad;       // learned boundary: width of the low-order pad (0: CPML)
+    float lb_beta;  // learned boundary: increment damping at the outer edge of the pad
 };
 
 // Distance of cell (i, j) from the nearest edge of the padded grid.

@@ -61,6 +61,15 @@ __global__ __launch_bounds__(256) void k_lpad_stress(FGrid g, FWave w, KModel m,
     w.gxx[k] = gxx; w.gzz[k] = gzz; w.gxz[k] = gxz; w.gzx[k] = gzx;
 }
 
+// Increment damping of the pad, 1 - beta ((lpad - d) / lpad)^2 at depth d: a
+// weak sponge that removes the slow growth a one-step-trained ring can show
+// over thousands of steps.  1 outside the pad.
+__device__ __forceinline__ float lb_gamma(const FGrid &g, int d) {
+    if (d >= g.lpad) return 1.f;
+    const float r = (float)(g.lpad - d) / (float)g.lpad;
+    return 1.f - g.lb_beta * r * r;
+}
+
 // 2nd-order velocity update on the live pad cells (the network ring and the
 // ghost line are left to k_lb_ring / stay zero).  The P/S split is not carried
 // in the pad: nothing reads S there.
@@ -75,12 +84,13 @@ __global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel 
     const float *q = m.q;
     const int wb = 1 - rb;
     const float dt2 = g.dt2;
+    const float gam = lb_gamma(g, edge_dist(g, i, j));
     if (i >= 1 && i < g.Xn - 2 && j >= 1 && j < g.Zn - 1) { // vx
         const float Tx = (w.th[k + P] - w.th[k]) * g.idx;
         const float Wz = (w.om[k] - w.om[k - 1]) * g.idz;
         const float C = (q[k0] * w.gzx[k] - q[k0 - 1] * w.gzx[k - 1]) * g.idz;
         const float E = (q[k0 + P] * w.gzz[k + P] - q[k0] * w.gzz[k]) * g.idx;
-        const float d = w.DV[0][k] + dt2 * (Tx + Wz + 2.f * (C - E));
+        const float d = gam * (w.DV[0][k] + dt2 * (Tx + Wz + 2.f * (C - E)));
         w.DV[0][k] = d;
         w.V[wb][0][k] = w.V[rb][0][k] + d;
     }
@@ -89,7 +99,7 @@ __global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel 
         const float Wx = (w.om[k] - w.om[k - P]) * g.idx;
         const float B = (q[k0] * w.gxz[k] - q[k0 - P] * w.gxz[k - P]) * g.idx;
         const float A = (q[k0 + 1] * w.gxx[k + 1] - q[k0] * w.gxx[k]) * g.idz;
-        const float d = w.DV[1][k] + dt2 * (Tz - Wx + 2.f * (B - A));
+        const float d = gam * (w.DV[1][k] + dt2 * (Tz - Wx + 2.f * (B - A)));
         w.DV[1][k] = d;
         w.V[wb][1][k] = w.V[rb][1][k] + d;
     }
@@ -185,7 +195,7 @@ __global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, con
     float sg_out = 1.f;
     if (side == 1) sg_out = kLbXS[f];
     if (side == 3) sg_out = kLbZS[f];
-    const float d = w.DV[f][k] + sg_out * acc;
+    const float d = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
     w.DV[f][k] = d;
     w.V[wb][f][k] = w.V[rb][f][k] + d;
 }

@@ -63,6 +63,8 @@ def lib():
     L.ouc_set_graphs.restype = C.c_int
     L.ouc_set_learned_boundary.argtypes = [C.c_void_p, fp, C.c_int32]
     L.ouc_set_learned_boundary.restype = C.c_int
+    L.ouc_set_learned_damping.argtypes = [C.c_void_p, C.c_float]
+    L.ouc_set_learned_damping.restype = C.c_int
     L.ouc_get_learned_filter.argtypes = [C.c_void_p, fp]
     L.ouc_get_learned_filter.restype = C.c_int
     L.ouc_forward_snapshots.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, P(C.c_int32), C.c_int32, C.c_int32,
@@ -194,12 +196,12 @@ class Engine:
         return int(t.core_cols), int(t.prefetch)
 
     def autotune(self, nshots, reps=5):
-        """Times the 12 candidate tiles on this grid, keeps the fastest;
+        """Times the 18 candidate tiles on this grid, keeps the fastest;
         returns ((core_cols, prefetch), {(cols, pf): ms per launch})."""
         t = abi.Tile()
-        ms = np.zeros(12, np.float32)
+        ms = np.zeros(18, np.float32)
         self._check(lib().ouc_autotune(self._h, int(nshots), int(reps), C.byref(t), abi.fptr(ms)))
-        cands = [(c, p) for c in (32, 64, 128, 256) for p in (1, 2, 3)]
+        cands = [(c, p) for c in (8, 16, 32, 64, 128, 256) for p in (1, 2, 3)]
         return (int(t.core_cols), int(t.prefetch)), {k: float(v) for k, v in zip(cands, ms)}
 
     def set_graphs(self, on=True):
@@ -214,6 +216,10 @@ class Engine:
             return
         w = np.ascontiguousarray(weights, np.float32).ravel()
         self._check(lib().ouc_set_learned_boundary(self._h, abi.fptr(w), w.size))
+
+    def set_learned_damping(self, beta):
+        """Weak sponge of the learned pad (increment scaled by 1 - beta r^2)."""
+        self._check(lib().ouc_set_learned_damping(self._h, float(beta)))
 
     def learned_filter(self):
         """The composed filter [2][5][7][7] the ring kernel applies."""

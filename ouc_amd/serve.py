@@ -69,12 +69,17 @@ def model_for(grid, seed):
     return vp, vs
 
 
-def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, concurrent=True, warm=True,
+def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, schedule="sjf", warm=True,
           model_seed=11, mem_budget_gb=200.0, log=None):
     """Run all jobs; returns (per-job latency ms [len(jobs)], wall s, [GroupResult]).
-    Concurrent groups share `mem_budget_gb` of device memory: when all groups'
-    shots do not fit at once, every group's batch shrinks by the same factor
-    (the rest of its shots run as later batches)."""
+
+    schedule "sjf": groups run one after another, least work (cells x steps x
+    shots) first -- small grids finish early instead of queueing behind large
+    kernels; "concurrent": all groups at once, one host thread and stream each.
+    The groups' device buffers share `mem_budget_gb`: when all groups' shots do
+    not fit at once, every group's batch shrinks by the same factor (the rest
+    of its shots run as later batches)."""
+    concurrent = schedule == "concurrent"
     groups = {}
     for j in jobs:
         groups.setdefault(j.grid, []).append(j)
@@ -96,7 +101,7 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, co
     scale = min(1.0, mem_budget_gb * 1e9 / need) if need > 0 else 1.0
     for grid, v in engines.items():
         e, js, _, per_shot = v
-        cap = max(1, int(len(js) * scale)) if concurrent else len(js)
+        cap = max(1, int(len(js) * scale))
         e.set_batch_capacity(cap)
         tile = tuning.tuned(e, cap)[:2] if tune else e.tile()
         v[2] = tile
@@ -133,8 +138,11 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, co
             for t in th:
                 t.join()
         else:
-            for g in engines:
+            work = lambda g: g[0] * g[1] * sum(j.nt for j in engines[g][1])
+            for g in sorted(engines, key=work):
                 run(g, t_start)
+                if errors:
+                    break
         if errors:
             raise errors[0]
         return time.perf_counter() - t_start

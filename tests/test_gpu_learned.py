@@ -85,6 +85,7 @@ def test_ring_and_pad_kernels_follow_the_training_definition(LB, weights):
     t = 148
     with Engine(c.small_params(abi.BOUNDARY_LEARNED), c.vp, c.vs) as e:
         e.set_learned_boundary(weights)
+        e.set_learned_damping(0.0)  # the definition the training windows use
         snaps = e.forward_snapshots(c.shot(), [t - 2, t - 1, t], 0, 0, X, X).astype(np.float64)
         K = e.learned_filter().astype(np.float64)
     Vm, V, Vn = snaps
@@ -169,6 +170,25 @@ def test_trained_network_beats_the_band_without_it(LB, weights):
         assert r["learned"] < -12.0
         assert r["learned"] < r["no_network"] - 8.0
         assert r["learned"] < r["cpml_band"] - 8.0
+
+
+def test_long_runs_stay_stable(LB, weights):
+    """8000 steps (the longest serving shots) on the model whose undamped run
+    grew after ~4000: the default pad damping keeps the pad below the band
+    inside it, without the fallback."""
+    from ouc_amd import abi, serve
+    from ouc_amd.engine import Engine
+
+    grid = (1024, 512)
+    vp, vs = serve.model_for(grid, 12)
+    p = serve.group_params(grid, 8000, True, abi.STRIP_FP32)
+    p.boundary = abi.BOUNDARY_LEARNED  # divergence would raise
+    with Engine(p, vp, vs) as e:
+        e.set_learned_boundary(weights)
+        seis = e.forward_shot(dict(sx=500, sz=0, rec_z=0, fm=20.0))
+        ratio = e.timings()["boundary_ratio"]
+    print(f"8000 steps: pad / band amplitude ratio {ratio:.2f}")
+    assert np.isfinite(seis).all() and ratio < 2.0
 
 
 def test_divergence_falls_back_to_cpml(LB, weights):

@@ -34,18 +34,18 @@ def test_tiles_give_identical_results_and_tuner_picks_a_candidate(tmp_path, monk
     outs = {}
     with Engine(p, vp, vs) as e:
         assert e.tile() == (64, 2)
-        for tile in ((64, 2), (32, 1), (128, 3), (256, 2)):
+        for tile in ((64, 2), (8, 1), (16, 3), (32, 1), (128, 3), (256, 2)):
             e.set_tile(*tile)
             outs[tile] = e.rtm_batch(shots)["stack_pp"]
         monkeypatch.setenv("OUC_TILE_CACHE", str(tmp_path / "tiles.json"))
         cols, pf, cached = tuning.tuned(e, len(shots), reps=3)
         assert not cached and (cols, pf) == e.tile()
-        assert cols in (32, 64, 128, 256) and pf in (1, 2, 3)
+        assert cols in (8, 16, 32, 64, 128, 256) and pf in (1, 2, 3)
         cols2, pf2, cached2 = tuning.tuned(e, len(shots))
         assert cached2 and (cols2, pf2) == (cols, pf)
         table = tuning.load_cache()[tuning.shape_key(tuning.device_arch(), p, len(shots))]["ms_per_launch"]
         print("[autotune] ms per launch: " + ", ".join(f"{k} {v:.3f}" for k, v in table.items()), flush=True)
-        assert len(table) == 12 and all(v > 0 for v in table.values())
+        assert len(table) == 18 and all(v > 0 for v in table.values())
         outs["tuned"] = e.rtm_batch(shots)["stack_pp"]
     ref = outs[(64, 2)]
     for k, v in outs.items():
