@@ -323,9 +323,8 @@ struct Launch {
 Launch launches(const FusedState *st, int B) {
     const FGrid &f = st->g;
     Launch L{};
-    const int TZ = 256 - 4 * f.N;
     L.has_core = f.cx1 > f.cx0 && f.cz1 > f.cz0;
-    L.core = dim3((f.cx1 - f.cx0 + f.csx - 1) / f.csx, (f.cz1 - f.cz0 + TZ - 1) / TZ, B);
+    L.core = dim3((f.cx1 - f.cx0 + f.csx - 1) / f.csx, core_zblocks(f.cz0, f.cz1), B);
     L.band_s = dim3(st->ntiles_s, 1, B);
     L.band_v = dim3(st->ntiles_v, 1, B);
     L.recon_s = dim3(st->ntiles_rs, 1, B);
@@ -335,7 +334,7 @@ Launch launches(const FusedState *st, int B) {
     L.rg.cz0 = f.L + 2 * f.N; L.rg.cz1 = f.Zn - f.L - 2 * f.N;
     L.has_recon_core = L.rg.cx1 > L.rg.cx0 && L.rg.cz1 > L.rg.cz0;
     if (!L.has_recon_core) { L.rg.cx1 = L.rg.cx0; L.rg.cz1 = L.rg.cz0; }
-    L.recon_core = dim3((L.rg.cx1 - L.rg.cx0 + f.csx - 1) / f.csx, (L.rg.cz1 - L.rg.cz0 + TZ - 1) / TZ, B);
+    L.recon_core = dim3((L.rg.cx1 - L.rg.cx0 + f.csx - 1) / f.csx, core_zblocks(L.rg.cz0, L.rg.cz1), B);
     L.full = dim3((f.Zn + 63) / 64, (f.Xn + 3) / 4, B);
     return L;
 }
@@ -995,8 +994,7 @@ void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_
         for (int b = 0; b < 3; ++b, ++k) {
             FGrid f = st->g;
             f.csx = cols[a];
-            const int TZ = 256 - 4 * f.N;
-            dim3 grid((f.cx1 - f.cx0 + f.csx - 1) / f.csx, (f.cz1 - f.cz0 + TZ - 1) / TZ, nshots);
+            dim3 grid((f.cx1 - f.cx0 + f.csx - 1) / f.csx, core_zblocks(f.cz0, f.cz1), nshots);
             FStep st0{};
             if (f.cx1 <= f.cx0) { if (ms) ms[k] = 0; continue; }
             OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->p2.f(), c->q2.f(), st0, FRecon{})); // warm

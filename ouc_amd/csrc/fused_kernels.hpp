@@ -130,7 +130,7 @@ __device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k
 }
 
 // ---------------------------------------------------------------------------
-// Core kernel.  Block = 256 lanes along z covering TZ = 256 - 4N output rows
+// Core kernel.  Block = 256 lanes along z covering TZ = 224 output rows
 // plus a 2N halo each side; the block streams along x over SX columns.  At
 // stream position c it loads V at column c, forms the stress at column c - N
 // (x derivatives from a register queue of V, z derivatives through LDS), and
@@ -183,6 +183,15 @@ __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const 
     vz = w.V[rb][1][k];
 }
 
+// Output rows of a core block: 224 = 7 lines of 32 floats (128 B), with block
+// origins on line boundaries, so every column's state reads and writes cover
+// whole lines; the 2N halo rows each side fit in the 256 lanes for N <= 8.
+constexpr int kCoreTZ = 224;
+__host__ __device__ constexpr int core_zalign(int cz0) { return cz0 & ~31; }
+__host__ __device__ constexpr int core_zblocks(int cz0, int cz1) {
+    return cz1 > cz0 ? (cz1 - core_zalign(cz0) + kCoreTZ - 1) / kCoreTZ : 0;
+}
+
 constexpr int kCoreSX = 64; // default columns per block
 constexpr int kCorePF = 2;  // default columns of loads in flight ahead of use
 
@@ -194,18 +203,19 @@ constexpr int kCorePF = 2;  // default columns of loads in flight ahead of use
 template <int N, int RECON, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float *__restrict__ p2,
                                               const float *__restrict__ q2, FStep a, FRecon rc) {
-    constexpr int TZ = 256 - 4 * N;
+    constexpr int TZ = kCoreTZ;
+    static_assert(TZ + 4 * N <= 256, "core block too small for the halo");
     constexpr int Q = 2 * N + 1;
-    __shared__ float lv[2][256];
-    __shared__ float ls[5][256];
+    __shared__ float lv[2][2][256]; // [column parity][vx, vz][lane]
+    __shared__ float ls[2][5][256]; // [column parity][theta, omega, gzx, gxx, q][lane]
     const int l = threadIdx.x;
-    const int z = g.cz0 + blockIdx.y * TZ - 2 * N + l;
+    const int z = core_zalign(g.cz0) + blockIdx.y * TZ - 2 * N + l;
     const int xs = g.cx0 + blockIdx.x * g.csx;
     const int xe = min(xs + g.csx, g.cx1);
     const long so = (long)blockIdx.z * g.plane;
     const float *c = g.c;
     const int wb = 1 - a.rb;
-    const bool z_out = z >= g.cz0 && z < g.cz1 && l >= 2 * N && l < 256 - 2 * N;
+    const bool z_out = z >= g.cz0 && z < g.cz1 && l >= 2 * N && l < 2 * N + TZ;
     const bool z_st = l >= N && l < 256 - N && z >= N && z < g.Zn - N;
     KStrips rs{};
     if (RECON) rs = shot_strips(rc);
@@ -216,12 +226,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
     float qvx[Q], qvz[Q];                         // V at columns cc-2N..cc
     float xth[Q], xom[Q], xgzz[Q], xgxz[Q], xq[Q]; // stress at columns cs-2N..cs
-    float zq[N + 1][4];                           // z-terms Tz, Wz, KXz, KZz, delayed N
+    float zq[N][4];                               // z-terms Tz, Wz, KXz, KZz, delayed N - 1
     float pv[PF][2], ppq[PF][2], pst[PF][6];      // prefetched V, (p, q), state
 #pragma unroll
     for (int m = 0; m < Q; ++m) { qvx[m] = qvz[m] = 0.f; xth[m] = xom[m] = xgzz[m] = xgxz[m] = xq[m] = 0.f; }
 #pragma unroll
-    for (int m = 0; m <= N; ++m) zq[m][0] = zq[m][1] = zq[m][2] = zq[m][3] = 0.f;
+    for (int m = 0; m < N; ++m) zq[m][0] = zq[m][1] = zq[m][2] = zq[m][3] = 0.f;
     auto load_pq = [&](int cs, float *o) {
         o[0] = o[1] = 0.f;
         if (z_st && cs >= xs - N && cs < xe + N) {
@@ -266,20 +276,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         vin(cc + PF, pv[PF - 1][0], pv[PF - 1][1]);
         load_pq(cc + PF - N, ppq[PF - 1]);
         load_state(cc + PF - 2 * N, pst[PF - 1]);
-        // 2. stress at column cs = cc - N
+        // 2. stress at column cs = cc - N.  LDS is double-buffered by column
+        // parity: one barrier per column makes this column's V and the previous
+        // column's stress visible; this column's stress is read after the next
+        // barrier, so the z-derivative terms lag one column.
         const int cs = cc - N;
-        lv[0][l] = qvx[N];
-        lv[1][l] = qvz[N];
+        const int b = cc & 1;
+        lv[b][0][l] = qvx[N];
+        lv[b][1][l] = qvz[N];
         __syncthreads();
         float th = 0.f, om = 0.f, gzx = 0.f, gxx = 0.f, qq = 0.f, gzz = 0.f, gxz = 0.f;
         if (cc >= xs && z_st) {
             float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
 #pragma unroll
             for (int m = 1; m <= N; ++m) {
-                s1 += c[m - 1] * (qvx[N + m - 1] - qvx[N - m]);    // D-x Vx
-                s2 += c[m - 1] * (qvz[N + m] - qvz[N - m + 1]);    // D+x Vz
-                s3 += c[m - 1] * (lv[0][l + m] - lv[0][l - m + 1]); // D+z Vx
-                s4 += c[m - 1] * (lv[1][l + m - 1] - lv[1][l - m]); // D-z Vz
+                s1 += c[m - 1] * (qvx[N + m - 1] - qvx[N - m]);          // D-x Vx
+                s2 += c[m - 1] * (qvz[N + m] - qvz[N - m + 1]);          // D+x Vz
+                s3 += c[m - 1] * (lv[b][0][l + m] - lv[b][0][l - m + 1]); // D+z Vx
+                s4 += c[m - 1] * (lv[b][1][l + m - 1] - lv[b][1][l - m]); // D-z Vz
             }
             gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
             qq = qq0;
@@ -287,18 +301,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (!RECON || rc.exact) th += src_term(g, a, cs, z);
             om = qq * (gxz - gzx);
         }
-        ls[0][l] = th; ls[1][l] = om; ls[2][l] = gzx; ls[3][l] = gxx; ls[4][l] = qq;
-        __syncthreads();
-        // z-derivative terms at column cs (q0 = q at the cell)
+        ls[b][0][l] = th; ls[b][1][l] = om; ls[b][2][l] = gzx; ls[b][3][l] = gxx; ls[b][4][l] = qq;
+        // z-derivative terms at column cs - 1 from the previous column's stress
+        // (q0 = q at the cell)
         float tz = 0.f, wz = 0.f, kxz = 0.f, kzz = 0.f;
-        if (cc >= xs && l >= 2 * N && l < 256 - 2 * N) {
-            const float q0 = qq;
+        if (cc > xs && l >= 2 * N && l < 256 - 2 * N) {
+            const int p = b ^ 1;
+            const float q0 = ls[p][4][l];
 #pragma unroll
             for (int m = 1; m <= N; ++m) {
-                tz += c[m - 1] * (ls[0][l + m] - ls[0][l - m + 1]);
-                wz += c[m - 1] * (ls[1][l + m - 1] - ls[1][l - m]);
-                kxz += c[m - 1] * (ls[2][l + m - 1] * (ls[4][l + m - 1] - q0) - ls[2][l - m] * (ls[4][l - m] - q0));
-                kzz += c[m - 1] * (ls[3][l + m] * (ls[4][l + m] - q0) - ls[3][l - m + 1] * (ls[4][l - m + 1] - q0));
+                tz += c[m - 1] * (ls[p][0][l + m] - ls[p][0][l - m + 1]);
+                wz += c[m - 1] * (ls[p][1][l + m - 1] - ls[p][1][l - m]);
+                kxz += c[m - 1] * (ls[p][2][l + m - 1] * (ls[p][4][l + m - 1] - q0) - ls[p][2][l - m] * (ls[p][4][l - m] - q0));
+                kzz += c[m - 1] * (ls[p][3][l + m] * (ls[p][4][l + m] - q0) - ls[p][3][l - m + 1] * (ls[p][4][l - m + 1] - q0));
             }
             tz *= g.idz; wz *= g.idz; kxz *= g.idz; kzz *= g.idz;
         }
@@ -307,9 +322,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             xth[m] = xth[m + 1]; xom[m] = xom[m + 1]; xgzz[m] = xgzz[m + 1]; xgxz[m] = xgxz[m + 1]; xq[m] = xq[m + 1];
         }
         xth[Q - 1] = th; xom[Q - 1] = om; xgzz[Q - 1] = gzz; xgxz[Q - 1] = gxz; xq[Q - 1] = qq;
+        // z-terms of column cs - 1 enter the queue; zq[0] is column cs - N = cu
 #pragma unroll
-        for (int m = 0; m < N; ++m) { zq[m][0] = zq[m + 1][0]; zq[m][1] = zq[m + 1][1]; zq[m][2] = zq[m + 1][2]; zq[m][3] = zq[m + 1][3]; }
-        zq[N][0] = tz; zq[N][1] = wz; zq[N][2] = kxz; zq[N][3] = kzz;
+        for (int m = 0; m < N - 1; ++m) { zq[m][0] = zq[m + 1][0]; zq[m][1] = zq[m + 1][1]; zq[m][2] = zq[m + 1][2]; zq[m][3] = zq[m + 1][3]; }
+        zq[N - 1][0] = tz; zq[N - 1][1] = wz; zq[N - 1][2] = kxz; zq[N - 1][3] = kzz;
         // 3. update at column cu = cc - 2N
         const int cu = cc - 2 * N;
         if (cu >= xs && cu < xe && z_out) {
