@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--cpu-sample-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--tile", default="", help="core tile COLSxPREFETCH (e.g. 256x1) instead of the autotuner's")
     ap.add_argument("--serve", action="store_true",
                     help="uneven serving batch instead: 32 shots on grids 512x256..4096x2048, nt 2000-8000, "
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
@@ -98,10 +99,15 @@ def main():
     if engine == abi.ENGINE_FUSED:
         from ouc_amd import tuning
 
-        cols, pf, cached = tuning.tuned(e, batch)
-        tile = {"core_cols": cols, "prefetch": pf, "from_cache": cached}
-        print(f"[bench] tile {cols} columns, prefetch {pf} ({'cache' if cached else 'autotuned'})",
-              file=sys.stderr, flush=True)
+        if a.tile:
+            cols, pf = (int(v) for v in a.tile.lower().split("x"))
+            e.set_tile(cols, pf)
+            src = "given"
+        else:
+            cols, pf, cached = tuning.tuned(e, batch)
+            src = "cache" if cached else "autotuned"
+        tile = {"core_cols": cols, "prefetch": pf, "source": src}
+        print(f"[bench] tile {cols} columns, prefetch {pf} ({src})", file=sys.stderr, flush=True)
     e.set_profiling(True)
     k = 0
     for w in range(a.warmup):

@@ -25,9 +25,22 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "kernels_plain.hpp"
 
 namespace ouc {
+
+// Calls f(std::integral_constant<int, r>) for r = 0 .. Q-1, unrolled.
+template <class F, int... Rs>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, Rs...>) {
+    (f(std::integral_constant<int, Rs>{}), ...);
+}
+template <int Q, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_seq(f, std::make_integer_sequence<int, Q>{});
+}
 
 // Geometry shared by the fused kernels.
 struct FGrid {
@@ -256,12 +269,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         load_state(cc - 2 * N, pst[f]);
     }
 
-    for (int cc = xs - 2 * N; cc < xe + 2 * N; ++cc) {
-        // 1. V at column cc (loaded PF columns ago); issue the loads of column cc + PF
+    // Where the registers allow (N <= 5, one column of prefetch) the V and
+    // stress queues rotate instead of shifting: the column loop is unrolled by
+    // Q and logical entry m (0 oldest, Q-1 newest) of a queue at unrolled step
+    // r lives in register slot (r + 1 + m) mod Q.  Elsewhere they shift.
+    constexpr bool ROT = N <= 5 && PF == 1;
+    const int cbeg = xs - 2 * N, cend = xe + 2 * N;
+    auto body = [&](auto rr, const int cc) {
+        constexpr int r = decltype(rr)::value;
+#define RS(m) (ROT ? ((r + 1 + (m)) % Q) : (m))
+        if constexpr (!ROT) {
 #pragma unroll
-        for (int m = 0; m < Q - 1; ++m) { qvx[m] = qvx[m + 1]; qvz[m] = qvz[m + 1]; }
-        qvx[Q - 1] = pv[0][0];
-        qvz[Q - 1] = pv[0][1];
+            for (int m = 0; m < Q - 1; ++m) { qvx[m] = qvx[m + 1]; qvz[m] = qvz[m + 1]; }
+        }
+        // 1. V at column cc (loaded PF columns ago); issue the loads of column cc + PF
+        qvx[RS(Q - 1)] = pv[0][0];
+        qvz[RS(Q - 1)] = pv[0][1];
         const float pp = ppq[0][0], qq0 = ppq[0][1];
         float st6[6];
 #pragma unroll
@@ -282,16 +305,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // barrier, so the z-derivative terms lag one column.
         const int cs = cc - N;
         const int b = cc & 1;
-        lv[b][0][l] = qvx[N];
-        lv[b][1][l] = qvz[N];
+        lv[b][0][l] = qvx[RS(N)];
+        lv[b][1][l] = qvz[RS(N)];
         __syncthreads();
         float th = 0.f, om = 0.f, gzx = 0.f, gxx = 0.f, qq = 0.f, gzz = 0.f, gxz = 0.f;
         if (cc >= xs && z_st) {
             float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
 #pragma unroll
             for (int m = 1; m <= N; ++m) {
-                s1 += c[m - 1] * (qvx[N + m - 1] - qvx[N - m]);          // D-x Vx
-                s2 += c[m - 1] * (qvz[N + m] - qvz[N - m + 1]);          // D+x Vz
+                s1 += c[m - 1] * (qvx[RS(N + m - 1)] - qvx[RS(N - m)]);          // D-x Vx
+                s2 += c[m - 1] * (qvz[RS(N + m)] - qvz[RS(N - m + 1)]);          // D+x Vz
                 s3 += c[m - 1] * (lv[b][0][l + m] - lv[b][0][l - m + 1]); // D+z Vx
                 s4 += c[m - 1] * (lv[b][1][l + m - 1] - lv[b][1][l - m]); // D-z Vz
             }
@@ -317,11 +340,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
             tz *= g.idz; wz *= g.idz; kxz *= g.idz; kzz *= g.idz;
         }
+        if constexpr (!ROT) {
 #pragma unroll
-        for (int m = 0; m < Q - 1; ++m) {
-            xth[m] = xth[m + 1]; xom[m] = xom[m + 1]; xgzz[m] = xgzz[m + 1]; xgxz[m] = xgxz[m + 1]; xq[m] = xq[m + 1];
+            for (int m = 0; m < Q - 1; ++m) {
+                xth[m] = xth[m + 1]; xom[m] = xom[m + 1]; xgzz[m] = xgzz[m + 1]; xgxz[m] = xgxz[m + 1]; xq[m] = xq[m + 1];
+            }
         }
-        xth[Q - 1] = th; xom[Q - 1] = om; xgzz[Q - 1] = gzz; xgxz[Q - 1] = gxz; xq[Q - 1] = qq;
+        xth[RS(Q - 1)] = th; xom[RS(Q - 1)] = om; xgzz[RS(Q - 1)] = gzz; xgxz[RS(Q - 1)] = gxz; xq[RS(Q - 1)] = qq;
         // z-terms of column cs - 1 enter the queue; zq[0] is column cs - N = cu
 #pragma unroll
         for (int m = 0; m < N - 1; ++m) { zq[m][0] = zq[m + 1][0]; zq[m][1] = zq[m + 1][1]; zq[m][2] = zq[m + 1][2]; zq[m][3] = zq[m + 1][3]; }
@@ -329,20 +354,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // 3. update at column cu = cc - 2N
         const int cu = cc - 2 * N;
         if (cu >= xs && cu < xe && z_out) {
-            const float q0 = xq[N];
+            const float q0 = xq[RS(N)];
             float tx = 0.f, wx = 0.f, kxx = 0.f, kzx = 0.f;
 #pragma unroll
             for (int m = 1; m <= N; ++m) {
-                tx += c[m - 1] * (xth[N + m] - xth[N - m + 1]);
-                wx += c[m - 1] * (xom[N + m - 1] - xom[N - m]);
-                kxx += c[m - 1] * (xgzz[N + m] * (xq[N + m] - q0) - xgzz[N - m + 1] * (xq[N - m + 1] - q0));
-                kzx += c[m - 1] * (xgxz[N + m - 1] * (xq[N + m - 1] - q0) - xgxz[N - m] * (xq[N - m] - q0));
+                tx += c[m - 1] * (xth[RS(N + m)] - xth[RS(N - m + 1)]);
+                wx += c[m - 1] * (xom[RS(N + m - 1)] - xom[RS(N - m)]);
+                kxx += c[m - 1] * (xgzz[RS(N + m)] * (xq[RS(N + m)] - q0) - xgzz[RS(N - m + 1)] * (xq[RS(N - m + 1)] - q0));
+                kzx += c[m - 1] * (xgxz[RS(N + m - 1)] * (xq[RS(N + m - 1)] - q0) - xgxz[RS(N - m)] * (xq[RS(N - m)] - q0));
             }
             tx *= g.idx; wx *= g.idx; kxx *= g.idx; kzx *= g.idx;
             const float Kx = zq[0][2] - kxx, Kz = kzx - zq[0][3];
             const float hx = zq[0][1], hz = -wx;
             const long k = so + (long)cu * g.P + z;
-            float vx = qvx[0], vz = qvz[0];
+            float vx = qvx[RS(0)], vz = qvz[RS(0)];
             if (a.inj && z == a.zr) { vx = w.V[a.rb][0][k]; vz = w.V[a.rb][1][k]; } // state, not the injected record
             const float dt2 = g.dt2;
             const float sg = RECON ? -1.f : 1.f; // the reverse step runs the increments backwards
@@ -361,6 +386,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 correlate(g, a, k, cu, z, vx, vz, sx, sz);
             }
         }
+#undef RS
+    };
+    if constexpr (ROT) {
+        for (int c0 = cbeg; c0 < cend; c0 += Q)
+            static_for<Q>([&](auto rr) {
+                const int cc = c0 + decltype(rr)::value;
+                if (cc < cend) body(rr, cc);
+            });
+    } else {
+        for (int cc = cbeg; cc < cend; ++cc) body(std::integral_constant<int, 0>{}, cc);
     }
 }
 
