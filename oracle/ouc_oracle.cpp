@@ -401,17 +401,36 @@ void decode_chunk(const uint16_t *h, int n, int8_t e, float *x) {
     for (int k = 0; k < n; ++k) x[k] = std::ldexp(f16_to_f32(h[k]), (int)e);
 }
 
+/* One chunk of one slot with error feedback: the encoder codes x = v + r,
+ * where r is the cell's quantization error at the previous slot of the same
+ * field, and keeps r' = x - decode(x) (exact in FP32: decode(x) is x rounded
+ * to 11 significant bits).  Decoded values are v^t + r^{t-1} - r^t, so the
+ * error is the first time difference of a sequence bounded by half a unit in
+ * the last place.  A non-finite r' restarts the feedback at zero. */
+void encode_chunk_fb(const float *v, int n, float *r, uint16_t *h, int8_t *e_out, float *decoded) {
+    float x[64];
+    for (int k = 0; k < n; ++k) x[k] = v[k] + r[k];
+    encode_chunk(x, n, h, e_out);
+    decode_chunk(h, n, *e_out, decoded);
+    for (int k = 0; k < n; ++k) {
+        float d = x[k] - decoded[k];
+        r[k] = std::isfinite(d) ? d : 0.f;
+    }
+}
+
 template <class R>
 struct Strips {
     Ring ring;
     int nslots = 0;
     bool fp16 = false;
     std::vector<R> data; /* [slot][field][ring] */
+    std::vector<float> err; /* FP16 encoder error state [field][ring]; slots are saved in time order */
     void init(const Geo &g, int slots, bool use_fp16) {
         ring.init(g);
         nslots = slots;
         fp16 = use_fp16;
         data.assign((size_t)slots * RS_NF * ring.ncells, (R)0);
+        err.assign(use_fp16 ? (size_t)RS_NF * ring.ncells : 0, 0.f);
     }
     R *slot(int t, int f) { return data.data() + ((size_t)t * RS_NF + f) * ring.ncells; }
     void save(int t, const Wave<R> &W) {
@@ -425,15 +444,15 @@ struct Strips {
             }
             if (fp16) {
                 /* quantize in place: the FP32 value goes through the codec */
-                std::vector<float> tmp(64);
-                std::vector<uint16_t> h(64);
+                float v[64], dec[64];
+                uint16_t h[64];
+                float *r = err.data() + (size_t)f * ring.ncells;
                 for (size_t c0 = 0; c0 < ring.ncells; c0 += 64) {
                     int n = (int)std::min<size_t>(64, ring.ncells - c0);
-                    for (int k = 0; k < n; ++k) tmp[k] = (float)d[c0 + k];
+                    for (int k = 0; k < n; ++k) v[k] = (float)d[c0 + k];
                     int8_t e;
-                    encode_chunk(tmp.data(), n, h.data(), &e);
-                    decode_chunk(h.data(), n, e, tmp.data());
-                    for (int k = 0; k < n; ++k) d[c0 + k] = (R)tmp[k];
+                    encode_chunk_fb(v, n, r + c0, h, &e, dec);
+                    for (int k = 0; k < n; ++k) d[c0 + k] = (R)dec[k];
                 }
             }
         }
@@ -771,12 +790,19 @@ int oracle_ring(const ouc_params *p, int *ij, int64_t *ncells) {
     return OUC_OK;
 }
 
-/* Scaled-FP16 codec over n floats in chunks of 64. */
-int oracle_fp16_encode(const float *x, int64_t n, uint16_t *h, int8_t *e) {
-    for (int64_t c0 = 0, ci = 0; c0 < n; c0 += 64, ++ci) {
-        int k = (int)std::min<int64_t>(64, n - c0);
-        encode_chunk(x + c0, k, h + c0, e + ci);
-    }
+/* Scaled-FP16 strip codec: `steps` slots of one field, x as [steps][n], in
+ * chunks of 64 with the error feedback carried from slot to slot (zero at the
+ * first).  h is [steps][n], e is [steps][ceil(n/64)]. */
+int oracle_fp16_encode(const float *x, int64_t n, int32_t steps, uint16_t *h, int8_t *e) {
+    if (!x || n < 1 || steps < 1 || !h || !e) return OUC_INVALID_ARG;
+    const int64_t nch = (n + 63) / 64;
+    std::vector<float> r((size_t)n, 0.f);
+    float dec[64];
+    for (int32_t t = 0; t < steps; ++t)
+        for (int64_t c0 = 0, ci = 0; c0 < n; c0 += 64, ++ci) {
+            int k = (int)std::min<int64_t>(64, n - c0);
+            encode_chunk_fb(x + (size_t)t * n + c0, k, r.data() + c0, h + (size_t)t * n + c0, e + t * nch + ci, dec);
+        }
     return OUC_OK;
 }
 int oracle_fp16_decode(const uint16_t *h, const int8_t *e, int64_t n, float *x) {

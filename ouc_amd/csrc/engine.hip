@@ -83,6 +83,7 @@ ouc::KStrips ouc_ctx::kstrips() const {
     s.f32 = strips32.f();
     s.f16 = strips16.as<__half>();
     s.e8 = stripse8.as<int8_t>();
+    s.err = stripserr.f();
     s.ncells = ring.ncells;
     s.nchunks = strip_chunks;
     s.fp16 = p.strip_precision == OUC_STRIP_FP16;
@@ -93,11 +94,12 @@ void ouc_ctx::ensure_strips() {
     int want16 = p.strip_precision == OUC_STRIP_FP16;
     int64_t slots = g.nt - 1;
     if (strip_fp16 == want16 && strip_slots == slots) return;
-    strips32.release(); strips16.release(); stripse8.release();
+    strips32.release(); strips16.release(); stripse8.release(); stripserr.release();
     strip_chunks = (ring.ncells + 63) / 64;
     if (want16) {
         strips16.alloc((size_t)slots * RS_NF * ring.ncells * sizeof(__half));
         stripse8.alloc((size_t)slots * RS_NF * strip_chunks);
+        stripserr.alloc((size_t)RS_NF * ring.ncells * sizeof(float));
     } else {
         strips32.alloc((size_t)slots * RS_NF * ring.ncells * sizeof(float));
     }
@@ -218,6 +220,7 @@ void run_forward_plain(ouc_ctx *c, const ouc_shot *s, const std::vector<float> &
     hipStream_t st = c->stream;
     const dim3 upd = tiles(g.Xn - 2 * N, g.Zn - 2 * N);
     const int rblocks = (int)((c->ring.ncells + 255) / 256);
+    if (rtm && ks.err) OUC_HIP(hipMemsetAsync(ks.err, 0, c->stripserr.bytes, st));
     for (int t = 0; t < g.nt; ++t) {
         OUC_DISPATCH_N(N, k_assemble<NN><<<upd, kTile, 0, st>>>(kg, w));
         if (rtm && t >= 1) k_strip_save_v<<<rblocks, 256, 0, st>>>(kr, ks, t - 1, w.Vx, w.Vz, g.P);
@@ -555,26 +558,30 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, floa
     return OUC_OK;
 }
 
-ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, uint16_t *half_bits, int8_t *exponents) {
-    if (!x || n < 1 || !half_bits || !exponents) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
+ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, int32_t steps, uint16_t *half_bits,
+                                int8_t *exponents) {
+    if (!x || n < 1 || steps < 1 || !half_bits || !exponents) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
     try {
         OUC_HIP(hipSetDevice(device));
         int64_t chunks = (n + 63) / 64;
-        DevBuf din, dh, de;
-        din.alloc(n * sizeof(float));
-        dh.alloc(n * sizeof(__half));
-        de.alloc(chunks);
-        OUC_HIP(hipMemcpy(din.ptr, x, n * sizeof(float), hipMemcpyHostToDevice));
+        DevBuf din, dh, de, derr;
+        din.alloc((size_t)steps * n * sizeof(float));
+        dh.alloc((size_t)steps * n * sizeof(__half));
+        de.alloc((size_t)steps * chunks);
+        derr.alloc(n * sizeof(float));
+        OUC_HIP(hipMemcpy(din.ptr, x, (size_t)steps * n * sizeof(float), hipMemcpyHostToDevice));
+        OUC_HIP(hipMemset(derr.ptr, 0, derr.bytes));
         KStrips s{};
         s.f16 = dh.as<__half>();
         s.e8 = de.as<int8_t>();
+        s.err = derr.f();
         s.ncells = n;
         s.nchunks = chunks;
         s.fp16 = 1;
-        k_codec_fp16<<<(unsigned)((n + 255) / 256), 256>>>(s, din.f(), n);
+        k_codec_fp16<<<(unsigned)((n + 255) / 256), 256>>>(s, din.f(), n, steps);
         OUC_HIP(hipGetLastError());
-        OUC_HIP(hipMemcpy(half_bits, dh.ptr, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-        OUC_HIP(hipMemcpy(exponents, de.ptr, chunks, hipMemcpyDeviceToHost));
+        OUC_HIP(hipMemcpy(half_bits, dh.ptr, (size_t)steps * n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        OUC_HIP(hipMemcpy(exponents, de.ptr, (size_t)steps * chunks, hipMemcpyDeviceToHost));
         return OUC_OK;
     } catch (const Error &e) {
         return fail(nullptr, e.code, e.what());

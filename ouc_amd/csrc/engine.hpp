@@ -77,7 +77,7 @@ struct ouc_ctx {
     ouc::DevBuf dX, dZ; // Xn
     ouc::DevBuf counts; // nx
     ouc::DevBuf img[4]; // pp, ps, pp_lap, ps_lap (nx*nz)
-    ouc::DevBuf strips32, strips16, stripse8;
+    ouc::DevBuf strips32, strips16, stripse8, stripserr; // stripserr: FP16 encoder error state [RS_NF][ring]
     int64_t strip_slots = 0, strip_chunks = 0;
     int strip_fp16 = -1;
     hipEvent_t ev[8] = {};

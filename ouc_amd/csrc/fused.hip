@@ -59,7 +59,7 @@ struct FusedState {
     FWave S{}, R{};
     DevBuf vp[3][2];       // source P velocity levels [cap][plane]
     DevBuf acc[3];         // npp, nps, den
-    DevBuf strips32, strips16, e8;
+    DevBuf strips32, strips16, e8, ef; // ef: FP16 encoder error state [cap][RS_NF][ring]
     long shot_strip = 0, shot_chunks = 0, nchunks = 0;
     int nslots = 0, fp16 = 0;
     DevBuf rec;            // [cap][2][nt][nx]
@@ -211,7 +211,7 @@ size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
     size_t b = 2 * kWaveArrays * plane + 6 * plane + 3 * plane;
     const size_t cells = (size_t)c->ring.ncells;
     b += (size_t)g.nt * RS_NF * cells * (fp16 ? sizeof(__half) : sizeof(float));
-    if (fp16) b += (size_t)g.nt * RS_NF * ((cells + 63) / 64);
+    if (fp16) b += (size_t)g.nt * RS_NF * ((cells + 63) / 64) + (size_t)RS_NF * cells * sizeof(float);
     b += (size_t)2 * g.nt * g.nx * sizeof(float) + (size_t)g.nt * sizeof(float) + g.nx * sizeof(int32_t);
     return b;
 }
@@ -238,10 +238,11 @@ void ensure(ouc_ctx *c, int shots) {
     st->nchunks = (c->ring.ncells + 63) / 64;
     st->shot_strip = (long)st->nslots * RS_NF * c->ring.ncells;
     st->shot_chunks = (long)st->nslots * RS_NF * st->nchunks;
-    st->strips32.release(); st->strips16.release(); st->e8.release();
+    st->strips32.release(); st->strips16.release(); st->e8.release(); st->ef.release();
     if (want16) {
         st->strips16.alloc((size_t)st->shot_strip * shots * sizeof(__half));
         st->e8.alloc((size_t)st->shot_chunks * shots);
+        st->ef.alloc((size_t)RS_NF * c->ring.ncells * shots * sizeof(float));
     } else {
         st->strips32.alloc((size_t)st->shot_strip * shots * sizeof(float));
     }
@@ -297,6 +298,7 @@ KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
     s.f32 = st->strips32.f();
     s.f16 = st->strips16.as<__half>();
     s.e8 = st->e8.as<int8_t>();
+    s.err = st->ef.f();
     s.ncells = c->ring.ncells;
     s.nchunks = st->nchunks;
     s.fp16 = st->fp16;
@@ -453,6 +455,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     const size_t span = (size_t)st->plane * B * sizeof(float);
     OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
     OUC_HIP(hipMemsetAsync(st->mon.ptr, 0, st->mon.bytes, s));
+    if (rtm && st->ef.ptr) OUC_HIP(hipMemsetAsync(st->ef.ptr, 0, st->ef.bytes, s));
     unsigned int *mon_f = st->mon.as<unsigned int>(), *mon_r = mon_f + 2 * st->cap;
     const KRing kr = c->kring();
     const KStrips ks0 = base_strips(st, c);
@@ -592,7 +595,8 @@ int default_capacity(ouc_ctx *c, int nshots) {
     OUC_HIP(hipMemGetInfo(&freeb, &total));
     // keep what this context already holds available for reuse
     FusedState *st = fs(c);
-    size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes;
+    size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes +
+                  st->ef.bytes;
     size_t avail = (size_t)((freeb + held) * 0.85);
     size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
     int cap = (int)std::max<size_t>(1, avail / per);

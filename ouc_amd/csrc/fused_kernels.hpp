@@ -672,6 +672,7 @@ __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, 
     if (s.f32) s.f32 += (long)blockIdx.z * shot_strip;
     if (s.f16) s.f16 += (long)blockIdx.z * shot_strip;
     if (s.e8) s.e8 += (long)blockIdx.z * shot_chunks;
+    if (s.err) s.err += (long)blockIdx.z * RS_NF * r.ncells;
     const long k = so + (long)i * g.P + j;
     if (with_v) {
         strip_put(s, slot, RS_VX, q, valid, valid ? w.V[vbuf][0][k] : 0.f);

@@ -304,10 +304,18 @@ enum { RS_VX = 0, RS_VZ, RS_TH, RS_OM, RS_GZX, RS_GZZ, RS_GXX, RS_GXZ, RS_NF };
 
 // FP32 and scaled-FP16 strip stores.  FP16: each 64-value chunk of one field
 // shares a power-of-two scale 2^e with max|x| * 2^-e in [2^14, 2^15).
+//
+// The FP16 encoder uses error feedback: it keeps, per field and ring cell, the
+// quantization error r of the previous slot and codes x = v + r instead of v,
+// then keeps r' = x - decode(x).  The decoded sequence is v^t + r^{t-1} - r^t,
+// so its error is the first time difference of a bounded sequence (noise
+// shaped away from the low frequencies the wavefield occupies).  Slots must
+// be stored in time order; `err` is zero before the first.
 struct KStrips {
     float *f32;          // [slot][RS_NF][ncells]
     __half *f16;         // [slot][RS_NF][ncells]
     int8_t *e8;          // [slot][RS_NF][nchunks]
+    float *err;          // [RS_NF][ncells] encoder error state (FP16 only)
     long ncells, nchunks;
     int fp16;
 };
@@ -333,9 +341,16 @@ __device__ __forceinline__ void strip_put(const KStrips &s, int slot, int f, lon
         if (valid) s.f32[base + q] = v;
         return;
     }
-    float mx = wave_max(valid ? fabsf(v) : 0.f);
+    float *ep = (valid && s.err) ? s.err + (long)f * s.ncells + q : nullptr;
+    const float x = ep ? v + *ep : v;
+    float mx = wave_max(valid ? fabsf(x) : 0.f);
     int e = scale_exponent(mx);
-    if (valid) s.f16[base + q] = __float2half_rn(ldexpf(v, -e));
+    const __half h = __float2half_rn(ldexpf(x, -e));
+    if (valid) s.f16[base + q] = h;
+    if (ep) {
+        const float r = x - ldexpf(__half2float(h), e); // exact: h is x rounded to 11 bits
+        *ep = isfinite(r) ? r : 0.f;
+    }
     if (valid && (q & 63) == 0) s.e8[((long)slot * RS_NF + f) * s.nchunks + (q >> 6)] = (int8_t)e;
 }
 
@@ -346,11 +361,18 @@ __device__ __forceinline__ float strip_get(const KStrips &s, int slot, int f, lo
     return ldexpf(__half2float(s.f16[base + q]), e);
 }
 
-// The strip codec alone (slot 0, field 0 of a single-field store).
-static __global__ __launch_bounds__(256) void k_codec_fp16(KStrips s, const float *__restrict__ x, long n) {
+// The strip codec alone: `steps` consecutive slots of one field, x and the
+// outputs as [steps][n] (s.err, when set, carries the feedback across them).
+static __global__ __launch_bounds__(256) void k_codec_fp16(KStrips s, const float *__restrict__ x, long n,
+                                                           int steps) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = q < n;
-    strip_put(s, 0, 0, q, valid, valid ? x[q] : 0.f);
+    for (int t = 0; t < steps; ++t) {
+        KStrips st = s;
+        st.f16 = s.f16 + (long)t * n;
+        st.e8 = s.e8 + (long)t * s.nchunks;
+        strip_put(st, 0, 0, q, valid, valid ? x[(long)t * n + q] : 0.f);
+    }
 }
 
 // Velocity part of slot `slot` from the assembled totals.  Launched with

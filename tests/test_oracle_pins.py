@@ -176,6 +176,61 @@ def test_chunk_codec_scale_and_error(oracle):
         assert np.all(err <= bound * (1 + 1e-6))
 
 
+def _decode_seq(oracle, h, e):
+    return np.stack([oracle.fp16_decode(h[t], e[t]) for t in range(h.shape[0])])
+
+
+def test_error_feedback_codec_telescopes(oracle):
+    """Successive slots of a smooth signal: with error feedback the running
+    sum of the decoding errors is minus the last slot's residual, so it stays
+    within half an FP16 unit of the coded value however many slots are summed,
+    while independent rounding (each slot coded alone) lets it random-walk."""
+    rng = np.random.default_rng(11)
+    T, n = 400, 64 * 6 + 5
+    t = np.arange(T)[:, None] * 0.0004
+    f = rng.uniform(10, 40, n)[None, :]
+    amp = 10.0 ** rng.uniform(-6, 2, n)[None, :]
+    x = (amp * np.sin(2 * np.pi * f * t + rng.uniform(0, 6, n)[None, :])).astype(np.float32)
+    h, e = oracle.fp16_encode(x)
+    y = _decode_seq(oracle, h, e)
+    h1, e1 = oracle.fp16_encode(x[0])
+    assert np.array_equal(h[0], h1) and np.array_equal(e[0], e1)  # zero feedback at the first slot
+    # half a binary16 unit at the coded value x_t = v_t + r_{t-1} (or the
+    # subnormal floor) times the chunk scale, with |r_{t-1}| <= half_ulp[t-1]
+    floor = np.stack([np.repeat(2.0 ** (e[k].astype(np.float64) - 25), 64)[:n] for k in range(T)])
+    half_ulp = np.zeros((T, n))
+    prev = np.zeros(n)
+    for k in range(T):
+        prev = half_ulp[k] = np.maximum((np.abs(x[k].astype(np.float64)) + prev) * 2.0 ** -11 * (1 + 1e-6), floor[k])
+    err = y.astype(np.float64) - x
+    run = np.cumsum(err, axis=0)
+    # run_t = -r_t (the residual of slot t) plus the FP32 roundings of the sums v_s + r_{s-1}
+    fp32_adds = np.cumsum((np.abs(x.astype(np.float64)) + half_ulp) * 2.0 ** -24, axis=0)
+    assert np.all(np.abs(run) <= half_ulp + fp32_adds)
+    assert np.all(np.abs(err[1:]) <= half_ulp[1:] + half_ulp[:-1])  # each slot within one unit
+    indep = np.stack([oracle.fp16_decode(*oracle.fp16_encode(x[k])) for k in range(T)]).astype(np.float64) - x
+    walk = np.max(np.abs(np.cumsum(indep, axis=0)), axis=0) / np.max(half_ulp, axis=0)
+    assert np.median(walk) > 3.0
+
+
+def test_error_feedback_lowers_fp16_strip_image_error(oracle):
+    """FP64 oracle, FP16 strips against FP32 strips, same shot: the
+    feedback keeps the image within 1e-3 below row 8 (independent rounding of
+    every slot gives about 3e-2 on this case)."""
+    from ouc_amd import velocity_models as vm
+    nx, nz, nt = 160, 96, 900
+    vp, vs, _ = vm.layered_random(nx, nz, seed=3)
+    p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, strip_precision=abi.STRIP_FP32)
+    shot = dict(sx=80, sz=0, rec_z=0, fm=22.0, mute=1)
+    ref = oracle.rtm_shot(p, vp, vs, shot, 64)
+    p.strip_precision = abi.STRIP_FP16
+    got = oracle.rtm_shot(p, vp, vs, shot, 64)
+    for k in ["img_pp", "img_ps"]:
+        err = np.linalg.norm(got[k][:, 8:] - ref[k][:, 8:]) / np.linalg.norm(ref[k][:, 8:])
+        print(f"[fp16 strips, FP64 oracle] {k} below row 8 rel-L2 {err:.2e}")
+        assert err < 1e-3, (k, err)
+
+
 # ---- strip ring ---------------------------------------------------------------
 
 @pytest.mark.parametrize("N", [3, 5])
