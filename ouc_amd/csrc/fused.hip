@@ -36,17 +36,27 @@ namespace {
 
 // k_core with the prefetch depth chosen at run time (1..3 for N = 5, 6; the
 // default elsewhere, to bound the number of instantiations).
-template <int N, int RECON>
-void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2, const float *q2,
-                 const FStep &a, const FRecon &rc) {
+template <int N, int MODE>
+void launch_core_mode(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2,
+                      const float *q2, const FStep &a, const FRecon &rc) {
     if constexpr (N == 5 || N == 6) {
         switch (pf) {
-        case 1: k_core<N, RECON, 1><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
-        case 3: k_core<N, RECON, 3><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
+        case 1: k_core<N, MODE, 1><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
+        case 3: k_core<N, MODE, 3><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
         default: break;
         }
     }
-    k_core<N, RECON, kCorePF><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc);
+    k_core<N, MODE, kCorePF><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc);
+}
+
+// RECON = 1: the reconstruction; otherwise a wavefield step, with the imaging
+// condition when the step carries imaging operands (a.spx).
+template <int N, int RECON>
+void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2, const float *q2,
+                 const FStep &a, const FRecon &rc) {
+    if constexpr (RECON) launch_core_mode<N, CORE_RECON>(pf, grid, s, f, w, p2, q2, a, rc);
+    else if (a.spx) launch_core_mode<N, CORE_IMAGE>(pf, grid, s, f, w, p2, q2, a, rc);
+    else launch_core_mode<N, CORE_STEP>(pf, grid, s, f, w, p2, q2, a, rc);
 }
 
 constexpr int kWaveArrays = 28; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O

@@ -208,14 +208,32 @@ __host__ __device__ constexpr int core_zblocks(int cz0, int cz1) {
 constexpr int kCoreSX = 64; // default columns per block
 constexpr int kCorePF = 2;  // default columns of loads in flight ahead of use
 
-// RECON = 1 runs the source reconstruction (reshot_u + reshot_s,
-// P-S-seperating.cu:890-1042) through the same pipeline: ring totals come from
-// the strips, the step is the reverse one, and the new source P velocity level
-// is written for the imaging condition.  Its core is [L+2N, Xn-L-2N)^2, so every
-// stress cell it forms lies off the ring.
-template <int N, int RECON, int PF>
+// Loads and stores at a 32-bit byte offset from a block-uniform base.  The core
+// addresses every array it streams as base + (column * P + z) * 4 with the
+// shot's plane folded into the base, so all arrays of one column share one
+// lane offset (scalar base + vector offset addressing, no 64-bit address
+// arithmetic per array).  Offsets stay below 4 GiB: one shot's plane.
+__device__ __forceinline__ float ldb(const float *base, unsigned boff) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
+}
+__device__ __forceinline__ void stb(float *base, unsigned boff, float v) {
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+// MODE 0 steps a wavefield (the forward source run, or the receiver run
+// without imaging); MODE 2 is the receiver run with the imaging condition
+// (its five imaging operands are prefetched with the state).  MODE 1 runs the
+// source reconstruction (reshot_u + reshot_s, P-S-seperating.cu:890-1042)
+// through the same pipeline: ring totals come from the strips, the step is the
+// reverse one, and the new source P velocity level is written for the imaging
+// condition.  Its core is [L+2N, Xn-L-2N)^2, so every stress cell it forms lies
+// off the ring.
+enum { CORE_STEP = 0, CORE_RECON = 1, CORE_IMAGE = 2 };
+template <int N, int MODE, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float *__restrict__ p2,
                                               const float *__restrict__ q2, FStep a, FRecon rc) {
+    constexpr bool RECON = MODE == CORE_RECON, IMG = MODE == CORE_IMAGE;
+    constexpr int NS = IMG ? 11 : 6; // prefetched per-cell operands: state (+ imaging)
     constexpr int TZ = kCoreTZ;
     static_assert(TZ + 4 * N <= 256, "core block too small for the halo");
     constexpr int Q = 2 * N + 1;
@@ -230,17 +248,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int wb = 1 - a.rb;
     const bool z_out = z >= g.cz0 && z < g.cz1 && l >= 2 * N && l < 2 * N + TZ;
     const bool z_st = l >= N && l < 256 - N && z >= N && z < g.Zn - N;
+    const bool z_vin = z >= g.vlo && z < g.Zn - g.vlo;
+    const bool z_inj = a.inj && z == a.zr;
+    // per-block bases (shot plane folded in) and lane byte offset
+    const unsigned P4 = (unsigned)g.P * 4u, zb = (unsigned)z * 4u;
+    auto off = [&](int col) { return (unsigned)col * P4 + zb; };
+    const float *const Vr0 = w.V[a.rb][0] + so, *const Vr1 = w.V[a.rb][1] + so;
+    float *const Vw0 = w.V[wb][0] + so, *const Vw1 = w.V[wb][1] + so;
+    float *const DV0 = w.DV[0] + so, *const DV1 = w.DV[1] + so;
+    float *const S0 = w.S[0] + so, *const S1 = w.S[1] + so;
+    float *const DS0 = w.DS[0] + so, *const DS1 = w.DS[1] + so;
+    // source cell and wavelet sample of this shot and step, read once
+    int src_i = -1, src_j = -1;
+    float src_v = 0.f;
+    if (a.src && (!RECON || rc.exact)) {
+        src_i = a.src[2 * blockIdx.z];
+        src_j = a.src[2 * blockIdx.z + 1];
+        src_v = a.wav[(long)blockIdx.z * g.nt + a.t];
+    }
+    // imaging condition on this step (shots that start late image only their own steps)
+    const bool img_on = IMG && !(a.t0 && a.t < a.t0[blockIdx.z]);
+    const float *const spx = img_on ? a.spx + so : nullptr, *const spz = img_on ? a.spz + so : nullptr;
+    float *const npp = img_on ? a.npp + so : nullptr, *const nps = img_on ? a.nps + so : nullptr;
+    float *const den = img_on ? a.den + so : nullptr;
     KStrips rs{};
     if (RECON) rs = shot_strips(rc);
     auto vin = [&](int i, float &vx, float &vz) {
-        if (RECON) recon_vin(g, w, rc, rs, a.rb, so, i, z, vx, vz);
-        else load_vin(g, w, a, so, i, z, vx, vz);
+        if (RECON) {
+            recon_vin(g, w, rc, rs, a.rb, so, i, z, vx, vz);
+            return;
+        }
+        vx = 0.f;
+        vz = 0.f;
+        if (!z_vin || i < g.vlo || i >= g.Xn - g.vlo) return;
+        const unsigned o = off(i);
+        vx = ldb(Vr0, o);
+        vz = ldb(Vr1, o);
+        if (z_inj && i >= g.L && i < g.Xn - g.L) { // load_record, P-S-seperating.cu:1471-1483
+            long io = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
+            vx = a.inj[io];
+            vz = a.inj[io + (long)g.nt * g.nx];
+        }
     };
 
     float qvx[Q], qvz[Q];                         // V at columns cc-2N..cc
     float xth[Q], xom[Q], xgzz[Q], xgxz[Q], xq[Q]; // stress at columns cs-2N..cs
     float zq[N][4];                               // z-terms Tz, Wz, KXz, KZz, delayed N - 1
-    float pv[PF][2], ppq[PF][2], pst[PF][6];      // prefetched V, (p, q), state
+    float pv[PF][2], ppq[PF][2], pst[PF][NS];     // prefetched V, (p, q), state (+ imaging)
 #pragma unroll
     for (int m = 0; m < Q; ++m) { qvx[m] = qvz[m] = 0.f; xth[m] = xom[m] = xgzz[m] = xgxz[m] = xq[m] = 0.f; }
 #pragma unroll
@@ -248,17 +302,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     auto load_pq = [&](int cs, float *o) {
         o[0] = o[1] = 0.f;
         if (z_st && cs >= xs - N && cs < xe + N) {
-            o[0] = p2[(long)cs * g.P + z];
-            o[1] = q2[(long)cs * g.P + z];
+            const unsigned k = off(cs);
+            o[0] = ldb(p2, k);
+            o[1] = ldb(q2, k);
         }
     };
     auto load_state = [&](int cu, float *o) {
         if (z_out && cu >= xs && cu < xe) {
-            const long k = so + (long)cu * g.P + z;
-            o[0] = w.DV[0][k]; o[1] = w.DV[1][k]; o[2] = w.S[0][k]; o[3] = w.S[1][k];
-            o[4] = w.DS[0][k]; o[5] = w.DS[1][k];
+            const unsigned k = off(cu);
+            o[0] = ldb(DV0, k); o[1] = ldb(DV1, k); o[2] = ldb(S0, k); o[3] = ldb(S1, k);
+            o[4] = ldb(DS0, k); o[5] = ldb(DS1, k);
+            if (IMG && img_on) {
+                o[6] = ldb(spx, k); o[7] = ldb(spz, k); o[8] = ldb(npp, k); o[9] = ldb(den, k); o[10] = ldb(nps, k);
+            }
         } else {
-            o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0.f;
+#pragma unroll
+            for (int e = 0; e < NS; ++e) o[e] = 0.f;
         }
     };
 #pragma unroll
@@ -286,15 +345,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         qvx[RS(Q - 1)] = pv[0][0];
         qvz[RS(Q - 1)] = pv[0][1];
         const float pp = ppq[0][0], qq0 = ppq[0][1];
-        float st6[6];
+        float st6[NS];
 #pragma unroll
-        for (int e = 0; e < 6; ++e) st6[e] = pst[0][e];
+        for (int e = 0; e < NS; ++e) st6[e] = pst[0][e];
 #pragma unroll
         for (int f = 0; f < PF - 1; ++f) {
             pv[f][0] = pv[f + 1][0]; pv[f][1] = pv[f + 1][1];
             ppq[f][0] = ppq[f + 1][0]; ppq[f][1] = ppq[f + 1][1];
 #pragma unroll
-            for (int e = 0; e < 6; ++e) pst[f][e] = pst[f + 1][e];
+            for (int e = 0; e < NS; ++e) pst[f][e] = pst[f + 1][e];
         }
         vin(cc + PF, pv[PF - 1][0], pv[PF - 1][1]);
         load_pq(cc + PF - N, ppq[PF - 1]);
@@ -321,7 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
             qq = qq0;
             th = pp * (gxx + gzz);
-            if (!RECON || rc.exact) th += src_term(g, a, cs, z);
+            if (!RECON || rc.exact) th += (cs == src_i && z == src_j) ? src_v : 0.f;
             om = qq * (gxz - gzx);
         }
         ls[b][0][l] = th; ls[b][1][l] = om; ls[b][2][l] = gzx; ls[b][3][l] = gxx; ls[b][4][l] = qq;
@@ -366,24 +425,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             tx *= g.idx; wx *= g.idx; kxx *= g.idx; kzx *= g.idx;
             const float Kx = zq[0][2] - kxx, Kz = kzx - zq[0][3];
             const float hx = zq[0][1], hz = -wx;
-            const long k = so + (long)cu * g.P + z;
+            const unsigned k = off(cu);
             float vx = qvx[RS(0)], vz = qvz[RS(0)];
-            if (a.inj && z == a.zr) { vx = w.V[a.rb][0][k]; vz = w.V[a.rb][1][k]; } // state, not the injected record
+            if (z_inj) { vx = ldb(Vr0, k); vz = ldb(Vr1, k); } // state, not the injected record
             const float dt2 = g.dt2;
             const float sg = RECON ? -1.f : 1.f; // the reverse step runs the increments backwards
             float d;
-            d = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); w.DV[0][k] = d; vx = vx + sg * d;
-            d = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); w.DV[1][k] = d; vz = vz + sg * d;
-            w.V[wb][0][k] = vx;
-            w.V[wb][1][k] = vz;
-            d = st6[4] + sg * dt2 * (hx + Kx); w.DS[0][k] = d; const float sx = st6[2] + sg * d; w.S[0][k] = sx;
-            d = st6[5] + sg * dt2 * (hz + Kz); w.DS[1][k] = d; const float sz = st6[3] + sg * d; w.S[1][k] = sz;
+            d = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); stb(DV0, k, d); vx = vx + sg * d;
+            d = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); stb(DV1, k, d); vz = vz + sg * d;
+            stb(Vw0, k, vx);
+            stb(Vw1, k, vz);
+            d = st6[4] + sg * dt2 * (hx + Kx); stb(DS0, k, d); const float sx = st6[2] + sg * d; stb(S0, k, sx);
+            d = st6[5] + sg * dt2 * (hz + Kz); stb(DS1, k, d); const float sz = st6[3] + sg * d; stb(S1, k, sz);
             if (RECON) {
-                rc.vpx[k] = vx - sx;
-                rc.vpz[k] = vz - sz;
+                stb(rc.vpx + so, k, vx - sx);
+                stb(rc.vpz + so, k, vz - sz);
             } else {
                 record(g, a, cu, z, vx, vz, sx, sz);
-                correlate(g, a, k, cu, z, vx, vz, sx, sz);
+                if (IMG && img_on) { // corr_v (P-S-seperating.cu:1426-1439); core cells are interior
+                    const float sa = st6[6 % NS], sb = st6[7 % NS];
+                    const float rpx = vx - sx, rpz = vz - sz;
+                    stb(npp, k, st6[8 % NS] + (sa * rpx + sb * rpz));
+                    stb(den, k, st6[9 % NS] + (sa * sa + sb * sb));
+                    stb(nps, k, st6[10 % NS] + (sa * sx + sb * sz));
+                }
             }
         }
 #undef RS
