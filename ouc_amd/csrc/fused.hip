@@ -14,6 +14,7 @@
 
 #include "engine.hpp"
 #include "fused.hpp"
+#include "edge_kernels.hpp"
 #include "fused_kernels.hpp"
 #include "lapfft.hpp"
 #include "lpad_kernels.hpp"
@@ -59,7 +60,7 @@ void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &
     else launch_core_mode<N, CORE_STEP>(pf, grid, s, f, w, p2, q2, a, rc);
 }
 
-constexpr int kWaveArrays = 28; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O
+constexpr int kWaveArrays = 32; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O, 4 F2
 
 struct FusedState {
     int cap = 0;           // shots the buffers hold
@@ -76,6 +77,9 @@ struct FusedState {
     DevBuf srcpos, wav, counts, t0;
     DevBuf tiles_s, tiles_v, tiles_rs, tiles_rv;
     int ntiles_s = 0, ntiles_v = 0, ntiles_rs = 0, ntiles_rv = 0;
+    bool edge = false;     // CPML band by the streaming k_edge (CPML contexts) instead of the tile kernels
+    EPlan eplan{};
+    int eblocks = 0;
     DevBuf img;            // [4][nx*nz] scratch for one shot's images
     // learned boundary
     DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
@@ -140,7 +144,8 @@ void wave_layout(DevBuf &mem, FWave &w, long plane, int cap) {
     float **slots[kWaveArrays] = {&w.V[0][0], &w.V[0][1], &w.V[1][0], &w.V[1][1], &w.DV[0], &w.DV[1], &w.S[0],
                                   &w.S[1],    &w.DS[0],   &w.DS[1],   &w.th,      &w.om,     &w.gxx,   &w.gxz,
                                   &w.gzx,     &w.gzz,     &w.F[0],    &w.F[1],    &w.F[2],   &w.F[3],  &w.O[0],
-                                  &w.O[1],    &w.O[2],    &w.O[3],    &w.O[4],    &w.O[5],   &w.O[6],  &w.O[7]};
+                                  &w.O[1],    &w.O[2],    &w.O[3],    &w.O[4],    &w.O[5],   &w.O[6],  &w.O[7],
+                                  &w.F2[0],   &w.F2[1],   &w.F2[2],   &w.F2[3]};
     for (int a = 0; a < kWaveArrays; ++a) *slots[a] = b + (size_t)a * span;
 }
 
@@ -173,6 +178,19 @@ std::vector<int2> tile_list(const FGrid &g, Pred want) {
                 for (int j = j0; j < std::min(j0 + 64, g.Zn - g.N) && !any; ++j) any = want(i, j);
             if (any) out.push_back(make_int2(i0, j0));
         }
+    return out;
+}
+
+// Orders a tile list for the round-robin dispatch of workgroups over the 8
+// XCDs (block b runs on XCD b mod 8): XCD x gets a contiguous run of the
+// spatially ordered list, so tiles that share stencil halos share that XCD's
+// L2 instead of fetching the halos once per XCD.  The list is padded to a
+// multiple of 8 with tiles far outside the grid, on which the band kernels
+// return at once.
+std::vector<int2> xcd_order(const std::vector<int2> &in) {
+    const size_t n8 = (in.size() + 7) / 8 * 8, per = n8 / 8;
+    std::vector<int2> out(n8, make_int2(-(1 << 20), -(1 << 20)));
+    for (size_t p = 0; p < in.size(); ++p) out[8 * (p % per) + p / per] = in[p];
     return out;
 }
 
@@ -267,10 +285,33 @@ void ensure(ouc_ctx *c, int shots) {
     auto deep = [&](int i, int j) {
         return i >= f.cx0 + g.N && i < f.cx1 - g.N && j >= f.cz0 + g.N && j < f.cz1 - g.N;
     };
-    std::vector<int2> ts = tile_list(f, [&](int i, int j) { return !deep(i, j); });
-    std::vector<int2> tv = tile_list(f, [&](int i, int j) { return !core(i, j); });
+    std::vector<int2> ts = xcd_order(tile_list(f, [&](int i, int j) { return !deep(i, j); }));
+    std::vector<int2> tv = xcd_order(tile_list(f, [&](int i, int j) { return !core(i, j); }));
     st->ntiles_s = (int)ts.size();
     st->ntiles_v = (int)tv.size();
+    st->edge = f.lpad == 0;
+    if (st->edge) {
+        // band rectangles: top and bottom full width, left and right between them
+        const int N = g.N;
+        const int rx[4][4] = {{N, g.Xn - N, N, f.cz0},
+                              {N, g.Xn - N, f.cz1, g.Zn - N},
+                              {N, f.cx0, f.cz0, f.cz1},
+                              {f.cx1, g.Xn - N, f.cz0, f.cz1}};
+        EPlan &pl = st->eplan;
+        pl.xchunk = 64;
+        const int TZ = kEdgeW - 4 * N;
+        int nb = 0;
+        for (int r = 0; r < 4; ++r) {
+            ERect &R = pl.r[r];
+            R.x0 = rx[r][0]; R.x1 = rx[r][1]; R.z0 = rx[r][2]; R.z1 = rx[r][3];
+            const bool any = R.x1 > R.x0 && R.z1 > R.z0;
+            R.nzb = any ? (R.z1 - R.z0 + TZ - 1) / TZ : 1;
+            R.nxb = any ? (R.x1 - R.x0 + pl.xchunk - 1) / pl.xchunk : 0;
+            R.first = nb;
+            nb += R.nzb * R.nxb;
+        }
+        st->eblocks = nb;
+    }
     st->tiles_s.alloc(ts.size() * sizeof(int2));
     st->tiles_v.alloc(tv.size() * sizeof(int2));
     OUC_HIP(hipMemcpy(st->tiles_s.ptr, ts.data(), ts.size() * sizeof(int2), hipMemcpyHostToDevice));
@@ -293,6 +334,8 @@ void ensure(ouc_ctx *c, int shots) {
             if (s_any) trs.push_back(make_int2(i0, j0));
             if (v_any) trv.push_back(make_int2(i0, j0));
         }
+    trs = xcd_order(trs);
+    trv = xcd_order(trv);
     st->ntiles_rs = (int)trs.size();
     st->ntiles_rv = (int)trv.size();
     st->tiles_rs.alloc(trs.size() * sizeof(int2));
@@ -378,22 +421,36 @@ struct CoreProbe {
     }
 };
 
+// One wavefield step.  save_ring: this step's strips are saved afterwards (the
+// streaming band kernel then stores the stress of the ring cells).
 void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, const FStep &a, CoreProbe &pr,
-                      int &nlaunch, unsigned int *mon) {
+                      int &nlaunch, unsigned int *mon, bool save_ring = false) {
     const FGrid &f = st->g;
     const KModel km = c->kmodel();
     hipStream_t s = c->stream;
     const int B = (int)L.core.z;
     if (f.lpad)
         k_lpad_stress<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a);
-    OUC_FDISPATCH(f.N, k_band_stress<NN><<<L.band_s, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_s.as<int2>(), a));
+    if (st->edge) {
+        EPlan pl = st->eplan;
+        pl.save_ring = save_ring ? 1 : 0;
+        if (a.spx) {
+            OUC_FDISPATCH(f.N, k_edge<NN, true><<<dim3(st->eblocks, 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
+        } else {
+            OUC_FDISPATCH(f.N, k_edge<NN, false><<<dim3(st->eblocks, 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
+        }
+    } else {
+        OUC_FDISPATCH(f.N, k_band_stress<NN><<<L.band_s, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_s.as<int2>(), a));
+    }
     if (L.has_core) {
         bool p = pr.want(nlaunch++);
         if (p) OUC_HIP(hipEventRecord(pr.ev[2 * pr.used], s));
         OUC_FDISPATCH(f.N, launch_core<NN, 0>(st->pf, L.core, s, f, w, c->p2.f(), c->q2.f(), a, FRecon{}));
         if (p) { OUC_HIP(hipEventRecord(pr.ev[2 * pr.used + 1], s)); ++pr.used; }
     }
-    OUC_FDISPATCH(f.N, k_band_velocity<NN><<<L.band_v, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_v.as<int2>(), a));
+    if (!st->edge) {
+        OUC_FDISPATCH(f.N, k_band_velocity<NN><<<L.band_v, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_v.as<int2>(), a));
+    }
     if (f.lpad) {
         // the ring reads the pad's increments before the pad update advances them
         const float sc = f.dt2 * f.idx;
@@ -485,7 +542,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     for (int t = 0; t < nt; ++t) {
         a.t = t;
         a.rb = rb;
-        wave_step_probed(c, st, L, st->S, a, probe, nlaunch, mon_f);
+        wave_step_probed(c, st, L, st->S, a, probe, nlaunch, mon_f, rtm);
         if (snap && ksnap < snap->n && snap->steps[ksnap] == t) {
             k_crop_v<<<dim3((snap->h + 255) / 256, snap->w), 256, 0, s>>>(
                 f, st->S.V[1 - rb][0], st->S.V[1 - rb][1], snap->x0, snap->z0, snap->w, snap->h,
@@ -493,7 +550,10 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             ++ksnap;
         }
         if (rtm) {
-            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks, t, 0,
+            FWave ws = st->S; // the stress memory variables of this step (the streaming band kernel double-buffers them)
+            if (st->edge && rb == 0)
+                for (int q = 0; q < 4; ++q) ws.F[q] = st->S.F2[q];
+            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, ws, kr, ks0, st->shot_strip, st->shot_chunks, t, 0,
                                                                0);
             if (t < nt - 1)
                 k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks,

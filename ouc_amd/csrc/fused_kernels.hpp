@@ -69,6 +69,7 @@ struct FWave {
     float *DV[2], *S[2], *DS[2];
     float *th, *om, *gxx, *gxz, *gzx, *gzz; // band stress (stored)
     float *F[4], *O[8];
+    float *F2[4];            // second F buffer (the streaming CPML band kernel double-buffers F)
 };
 
 // Per-step, per-batch arguments.
@@ -517,7 +518,10 @@ __global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m,
     w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
 }
 
-// Velocity with CPML on band cells.
+// Velocity with CPML on band cells.  Every array is addressed from a
+// block-uniform base (shot plane folded in) with a 32-bit lane byte offset, so
+// the z taps are immediate offsets of one address and the x taps share one
+// address per column offset across all fields.
 template <int N>
 __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
                                                        FStep a) {
@@ -526,8 +530,12 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
     if (!in_update(g, i, j) || in_core(g, i, j) || in_lpad(g, i, j)) return;
     const long so = (long)blockIdx.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
+    const unsigned o = (unsigned)k0 * 4u; // byte offset of the cell in a plane
+    const int P4 = g.P * 4;
     const float *c = g.c;
-    const int P = g.P;
+    const float *const om = w.om + so, *const th = w.th + so, *const gzx = w.gzx + so, *const gzz = w.gzz + so;
+    const float *const gxz = w.gxz + so, *const gxx = w.gxx + so;
+    const float *const F0 = w.F[0] + so, *const F1 = w.F[1] + so, *const F2 = w.F[2] + so, *const F3 = w.F[3] + so;
     const float *q = m.q; // per grid, no shot offset
     // Cg = D-z(q gzx), Eg = D+x(q gzz), Bg = D-x(q gxz), Ag = D+z(q gxx) and the
     // same of q F (the psi parts of the reference's du fields); KX, KZ are
@@ -539,53 +547,57 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
     // from the same operator.  Within N of the learned boundary's low-order pad
     // the g's do not, so the plain form (q0 = 0) is used there, as the
     // reconstruction uses it next to its ring.
-    const float q0 = (g.lpad && edge_dist(g, i, j) < g.lpad + N) ? 0.f : q[k0];
+    const float q0 = (g.lpad && edge_dist(g, i, j) < g.lpad + N) ? 0.f : ldb(q, o);
 #pragma unroll
     for (int mm = 1; mm <= N; ++mm) {
         const float cm = c[mm - 1];
-        const long xp = (long)mm * P, xm1 = (long)(mm - 1) * P;
-        Wx += cm * (w.om[k + xm1] - w.om[k - xp]);
-        Wz += cm * (w.om[k + mm - 1] - w.om[k - mm]);
-        Tx += cm * (w.th[k + xp] - w.th[k - xm1]);
-        Tz += cm * (w.th[k + mm] - w.th[k - mm + 1]);
+        const unsigned xp = o + (unsigned)(mm * P4), xm = o - (unsigned)(mm * P4);          // i + mm, i - mm
+        const unsigned xp1 = o + (unsigned)((mm - 1) * P4), xm1 = o - (unsigned)((mm - 1) * P4); // i + mm-1, i - (mm-1)
+        const unsigned zp = o + 4u * mm, zm = o - 4u * mm, zp1 = o + 4u * (mm - 1), zm1 = o - 4u * (mm - 1);
+        Wx += cm * (ldb(om, xp1) - ldb(om, xm));
+        Wz += cm * (ldb(om, zp1) - ldb(om, zm));
+        Tx += cm * (ldb(th, xp) - ldb(th, xm1));
+        Tz += cm * (ldb(th, zp) - ldb(th, zm1));
         {   // D-z along z: cells k+mm-1, k-mm
-            const float qa = q[k0 + mm - 1], qb = q[k0 - mm];
-            const float ga = w.gzx[k + mm - 1], gb = w.gzx[k - mm];
+            const float qa = ldb(q, zp1), qb = ldb(q, zm);
+            const float ga = ldb(gzx, zp1), gb = ldb(gzx, zm);
             KX += cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idz;
             Cg += cm * (ga * qa - gb * qb);
-            Cf += cm * (w.F[3][k + mm - 1] * qa - w.F[3][k - mm] * qb);
+            Cf += cm * (ldb(F3, zp1) * qa - ldb(F3, zm) * qb);
         }
         {   // D+x along x: cells k+mm*P, k-(mm-1)*P
-            const float qa = q[k0 + xp], qb = q[k0 - xm1];
-            const float ga = w.gzz[k + xp], gb = w.gzz[k - xm1];
+            const float qa = ldb(q, xp), qb = ldb(q, xm1);
+            const float ga = ldb(gzz, xp), gb = ldb(gzz, xm1);
             KX -= cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idx;
             Eg += cm * (ga * qa - gb * qb);
-            Ef += cm * (w.F[1][k + xp] * qa - w.F[1][k - xm1] * qb);
+            Ef += cm * (ldb(F1, xp) * qa - ldb(F1, xm1) * qb);
         }
         {   // D-x along x: cells k+(mm-1)*P, k-mm*P
-            const float qa = q[k0 + xm1], qb = q[k0 - xp];
-            const float ga = w.gxz[k + xm1], gb = w.gxz[k - xp];
+            const float qa = ldb(q, xp1), qb = ldb(q, xm);
+            const float ga = ldb(gxz, xp1), gb = ldb(gxz, xm);
             KZ += cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idx;
             Bg += cm * (ga * qa - gb * qb);
-            Bf += cm * (w.F[2][k + xm1] * qa - w.F[2][k - xp] * qb);
+            Bf += cm * (ldb(F2, xp1) * qa - ldb(F2, xm) * qb);
         }
         {   // D+z along z: cells k+mm, k-mm+1
-            const float qa = q[k0 + mm], qb = q[k0 - mm + 1];
-            const float ga = w.gxx[k + mm], gb = w.gxx[k - mm + 1];
+            const float qa = ldb(q, zp), qb = ldb(q, zm1);
+            const float ga = ldb(gxx, zp), gb = ldb(gxx, zm1);
             KZ -= cm * (ga * (qa - q0) - gb * (qb - q0)) * g.idz;
             Ag += cm * (ga * qa - gb * qb);
-            Af += cm * (w.F[0][k + mm] * qa - w.F[0][k - mm + 1] * qb);
+            Af += cm * (ldb(F0, zp) * qa - ldb(F0, zm1) * qb);
         }
     }
     Wx *= g.idx; Wz *= g.idz; Tx *= g.idx; Tz *= g.idz;
     const float Cp = (Cg - Cf) * g.idz, Ep = (Eg - Ef) * g.idx, Bp = (Bg - Bf) * g.idx, Ap = (Ag - Af) * g.idz;
     const float FX = Cf * g.idz - Ef * g.idx, FZ = Bf * g.idx - Af * g.idz;
-    const float a1 = m.ax[k0], a2 = m.az[k0];
+    const float a1 = ldb(m.ax, o), a2 = ldb(m.az, o);
     float OTx = 0.f, OTz = 0.f, OWz = 0.f, OWx = 0.f, OC = 0.f, OE = 0.f, OB = 0.f, OA = 0.f;
     if (a1 != 0.f || a2 != 0.f) {
         const float dt = g.dt;
-        OC = w.O[0][k]; OE = w.O[1][k]; OB = w.O[2][k]; OA = w.O[3][k];
-        OTx = w.O[4][k]; OTz = w.O[5][k]; OWz = w.O[6][k]; OWx = w.O[7][k];
+        float *const O0 = w.O[0] + so, *const O1 = w.O[1] + so, *const O2 = w.O[2] + so, *const O3 = w.O[3] + so;
+        float *const O4 = w.O[4] + so, *const O5 = w.O[5] + so, *const O6 = w.O[6] + so, *const O7 = w.O[7] + so;
+        OC = ldb(O0, o); OE = ldb(O1, o); OB = ldb(O2, o); OA = ldb(O3, o);
+        OTx = ldb(O4, o); OTz = ldb(O5, o); OWz = ldb(O6, o); OWx = ldb(O7, o);
         OC = OC + (a2 * Cp - a2 * OC) * dt;
         OE = OE + (a1 * Ep - a1 * OE) * dt;
         OB = OB + (a1 * Bp - a1 * OB) * dt;
@@ -594,20 +606,22 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
         OTz = OTz + (a2 * Tz - a2 * OTz) * dt;
         OWz = OWz + (a2 * Wz - a2 * OWz) * dt;
         OWx = OWx + (a1 * Wx - a1 * OWx) * dt;
-        w.O[0][k] = OC; w.O[1][k] = OE; w.O[2][k] = OB; w.O[3][k] = OA;
-        w.O[4][k] = OTx; w.O[5][k] = OTz; w.O[6][k] = OWz; w.O[7][k] = OWx;
+        stb(O0, o, OC); stb(O1, o, OE); stb(O2, o, OB); stb(O3, o, OA);
+        stb(O4, o, OTx); stb(O5, o, OTz); stb(O6, o, OWz); stb(O7, o, OWx);
     }
     const float Kx = (KX - FX) - OC + OE, Kz = (KZ - FZ) - OB + OA;
     const float hx = Wz - OWz, hz = -Wx + OWx;
     const float dt2 = g.dt2;
     const int wb = 1 - a.rb;
+    float *const DV0 = w.DV[0] + so, *const DV1 = w.DV[1] + so, *const S0 = w.S[0] + so, *const S1 = w.S[1] + so;
+    float *const DS0 = w.DS[0] + so, *const DS1 = w.DS[1] + so;
     float d;
-    d = w.DV[0][k] + dt2 * ((Tx - OTx) + hx + 2.f * Kx); w.DV[0][k] = d; const float vx = w.V[a.rb][0][k] + d;
-    d = w.DV[1][k] + dt2 * ((Tz - OTz) + hz + 2.f * Kz); w.DV[1][k] = d; const float vz = w.V[a.rb][1][k] + d;
-    w.V[wb][0][k] = vx;
-    w.V[wb][1][k] = vz;
-    d = w.DS[0][k] + dt2 * (hx + Kx); w.DS[0][k] = d; const float sx = w.S[0][k] + d; w.S[0][k] = sx;
-    d = w.DS[1][k] + dt2 * (hz + Kz); w.DS[1][k] = d; const float sz = w.S[1][k] + d; w.S[1][k] = sz;
+    d = ldb(DV0, o) + dt2 * ((Tx - OTx) + hx + 2.f * Kx); stb(DV0, o, d); const float vx = ldb(w.V[a.rb][0] + so, o) + d;
+    d = ldb(DV1, o) + dt2 * ((Tz - OTz) + hz + 2.f * Kz); stb(DV1, o, d); const float vz = ldb(w.V[a.rb][1] + so, o) + d;
+    stb(w.V[wb][0] + so, o, vx);
+    stb(w.V[wb][1] + so, o, vz);
+    d = ldb(DS0, o) + dt2 * (hx + Kx); stb(DS0, o, d); const float sx = ldb(S0, o) + d; stb(S0, o, sx);
+    d = ldb(DS1, o) + dt2 * (hz + Kz); stb(DS1, o, d); const float sz = ldb(S1, o) + d; stb(S1, o, sz);
     record(g, a, i, j, vx, vz, sx, sz);
     correlate(g, a, k, i, j, vx, vz, sx, sz);
 }
