@@ -178,9 +178,13 @@ ouc_status ouc_set_profiling(ouc_ctx *c, int32_t on);
 
 /* HIP graphs (default on): the fused engine captures each time loop of a
  * batch (forward, then receiver + reconstruction) with stream capture the
- * first time a batch size is run and replays the graph on later calls; eager
- * launches are used while profiling is on.  No counterpart in the reference,
- * which launches every kernel from the host loop (P-S-seperating.cu:2258-2364). */
+ * first time a batch size is run and replays the graph on later calls.  In
+ * the captured backward loop the reconstruction of each step runs on a second
+ * stream beside the receiver step (they share no data within a step) and the
+ * two are joined every step.  While profiling is on, the forward loop (which
+ * carries the sampled kernel timing) runs eagerly.  Off: every loop runs
+ * eagerly on one stream.  No counterpart in the reference, which launches
+ * every kernel from the host loop (P-S-seperating.cu:2258-2364). */
 ouc_status ouc_set_graphs(ouc_ctx *c, int32_t on);
 
 /* Shots per fused launch: ouc_rtm_batch runs at most `shots` shots together

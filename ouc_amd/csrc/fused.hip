@@ -100,7 +100,22 @@ struct FusedState {
         bool matches(int b, int r, int n) const { return exec && B == b && rtm == r && ncomp == n; }
     } gfwd, gbwd;
     void drop_graphs() { gfwd.drop(); gbwd.drop(); }
-    ~FusedState() { drop_graphs(); }
+    // side stream (default priority) for the reconstruction, which runs
+    // beside the receiver step of the same backward iteration; fork/join events
+    hipStream_t s2 = nullptr;
+    hipEvent_t evf = nullptr, evj = nullptr;
+    void ensure_side() {
+        if (s2) return;
+        OUC_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        OUC_HIP(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
+        OUC_HIP(hipEventCreateWithFlags(&evj, hipEventDisableTiming));
+    }
+    ~FusedState() {
+        drop_graphs();
+        if (evf) (void)hipEventDestroy(evf);
+        if (evj) (void)hipEventDestroy(evj);
+        if (s2) (void)hipStreamDestroy(s2);
+    }
 };
 
 // Enqueues `body` on stream s directly, or replays the graph it was captured
@@ -536,7 +551,11 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     a.zr = zr;
     a.rec = ncomp == 6 ? st->rec6.f() : st->rec.f();
     a.ncomp = ncomp;
+    // The forward loop holds the sampled k_core timing events, so it runs eagerly
+    // while profiling; the backward loop is captured either way.
     const bool graph = c->graphs && probe.ev.empty() && !snap && !rsnap;
+    const bool graph_b = c->graphs && !snap && !rsnap;
+    if (graph_b) st->ensure_side(); // the captured backward loop runs independent work on two streams
     int rb = 0, ksnap = 0;
     run_loop(s, graph, st->gfwd, B, rtm, ncomp, [&] {
     for (int t = 0; t < nt; ++t) {
@@ -593,8 +612,18 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     r.npp = st->acc[0].f(); r.nps = st->acc[1].f(); r.den = st->acc[2].f();
     r.t0 = st->t0.as<int>();
     int rbr = 0;
-    run_loop(s, graph, st->gbwd, B, rtm, ncomp, [&] {
+    // The reconstruction of iteration t writes source P level t and reads only
+    // the source state and the strips; the receiver step of the same iteration
+    // reads levels t+1 / t+2 (written in earlier iterations) and its own state.
+    // So the two run on two streams, joined at the end of every iteration.
+    // Eager launches (profiling) stay on one stream.
+    hipStream_t sr = graph_b ? st->s2 : s;
+    run_loop(s, graph_b, st->gbwd, B, rtm, ncomp, [&] {
     for (int t = nt - 1; t >= 0; --t) {
+        if (graph_b) {
+            OUC_HIP(hipEventRecord(st->evf, s));
+            OUC_HIP(hipStreamWaitEvent(sr, st->evf, 0));
+        }
         if (t < nt - 1) {
             // theta^{t+1} from V^{t+1}: ring totals from slot t+1 (FAITHFUL, one step late as the
             // reference) or slot t (EXACT); the hand-off step uses the forward's own state
@@ -614,15 +643,15 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             ra.rb = rbs;
             ra.src = st->srcpos.as<int>();
             ra.wav = st->wav.f();
-            OUC_FDISPATCH(f.N, k_recon_stress_f<NN><<<L.recon_s, dim3(64, 4), 0, s>>>(
+            OUC_FDISPATCH(f.N, k_recon_stress_f<NN><<<L.recon_s, dim3(64, 4), 0, sr>>>(
                                    f, st->S, c->p2.f(), c->q2.f(), rc, ra, st->tiles_rs.as<int2>()));
             if (L.has_recon_core)
-                OUC_FDISPATCH(f.N, launch_core<NN, 1>(st->pf, L.recon_core, s, L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
-            OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, s>>>(f, st->S, c->q2.f(), rc, ra,
-                                                                                        st->tiles_rv.as<int2>()));
+                OUC_FDISPATCH(f.N, launch_core<NN, 1>(st->pf, L.recon_core, sr, L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
+            OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, sr>>>(f, st->S, c->q2.f(), rc, ra,
+                                                                                         st->tiles_rv.as<int2>()));
             rbs = 1 - rbs;
             if (rsnap && krs < rsnap->n && rsnap->steps[krs] == t) { // reconstructed V^t
-                k_crop_v<<<dim3((rsnap->h + 255) / 256, rsnap->w), 256, 0, s>>>(
+                k_crop_v<<<dim3((rsnap->h + 255) / 256, rsnap->w), 256, 0, sr>>>(
                     f, st->S.V[rbs][0], st->S.V[rbs][1], rsnap->x0, rsnap->z0, rsnap->w, rsnap->h,
                     rsnap->dev + (size_t)krs * 2 * rsnap->w * rsnap->h);
                 ++krs;
@@ -637,6 +666,10 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         int nl2 = 0;
         wave_step_probed(c, st, L, st->R, r, none, nl2, mon_r);
         rbr = 1 - rbr;
+        if (graph_b) {
+            OUC_HIP(hipEventRecord(st->evj, sr));
+            OUC_HIP(hipStreamWaitEvent(s, st->evj, 0));
+        }
     }
     OUC_HIP(hipGetLastError());
     });
