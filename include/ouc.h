@@ -49,11 +49,14 @@ typedef enum ouc_status {
 enum { OUC_SEM_FAITHFUL = 0, OUC_SEM_EXACT = 1 };
 
 /* Storage of the boundary strips (P-S-seperating.cu:1852-1893 allocate them in
- * FP32).  FP16 stores each 64-value chunk as binary16 with a power-of-two
- * scale; compute stays FP32.  The FP16 encoder feeds each cell's quantization
- * error into the next slot it stores, so the error the reconstruction sees
- * is a first time difference (small at the wavefield's frequencies) rather
- * than white noise that the backward loop integrates. */
+ * FP32).  FP16 stores, per ring cell, the residual of each slot against the
+ * previous slot's decoded value, with the previous quantization error fed
+ * back, as binary16 with a power-of-two scale per 64-value chunk; every 32nd
+ * slot keeps an FP32 keyframe, and the reconstruction decodes backwards one
+ * slot per step.  The error the reconstruction sees is then a first time
+ * difference (small at the wavefield's frequencies) of a quantity about the
+ * size of one step's change times 2^-12, rather than white noise of 2^-12 of
+ * the value.  Compute stays FP32. */
 enum { OUC_STRIP_FP32 = 0, OUC_STRIP_FP16 = 1 };
 
 /* Absorbing boundary.  CPML is the reference's (P-S-seperating.cu:2067-2105).
@@ -228,12 +231,14 @@ ouc_status ouc_write_image_files(const char *dir, int32_t nx, int32_t nz, int32_
 /* Scaled-FP16 strip codec on the GPU, for bit-exact testing against a host
  * implementation: encodes `steps` consecutive slots of one field, x as
  * [steps][n] floats (host), in 64-value chunks exactly as the strip store
- * does, with the encoder's error feedback carried from slot to slot (starting
- * from zero).  Returns the binary16 bits [steps][n] and each chunk's
- * power-of-two exponent [steps][ceil(n/64)].  No counterpart in the
- * reference, whose strips are FP32 (P-S-seperating.cu:1852-1893). */
+ * does (predictive coding with error feedback, encoder state zero at the
+ * first slot, the last slot a keyframe).  Returns the binary16 residual bits
+ * [steps][n], each chunk's power-of-two exponent [steps][ceil(n/64)] and, if
+ * `decoded` is not NULL, the values [steps][n] decoded backwards as the
+ * reconstruction decodes them.  No counterpart in the reference, whose strips
+ * are FP32 (P-S-seperating.cu:1852-1893). */
 ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, int32_t steps, uint16_t *half_bits,
-                                int8_t *exponents);
+                                int8_t *exponents, float *decoded);
 
 /* Learned absorbing boundary (boundary LEARNED / LEARNED_FALLBACK; no
  * counterpart in the reference, whose only boundary is the CPML of

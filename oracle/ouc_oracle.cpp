@@ -401,70 +401,127 @@ void decode_chunk(const uint16_t *h, int n, int8_t e, float *x) {
     for (int k = 0; k < n; ++k) x[k] = std::ldexp(f16_to_f32(h[k]), (int)e);
 }
 
-/* One chunk of one slot with error feedback: the encoder codes x = v + r,
- * where r is the cell's quantization error at the previous slot of the same
- * field, and keeps r' = x - decode(x) (exact in FP32: decode(x) is x rounded
- * to 11 significant bits).  Decoded values are v^t + r^{t-1} - r^t, so the
- * error is the first time difference of a sequence bounded by half a unit in
- * the last place.  A non-finite r' restarts the feedback at zero. */
-void encode_chunk_fb(const float *v, int n, float *r, uint16_t *h, int8_t *e_out, float *decoded) {
-    float x[64];
-    for (int k = 0; k < n; ++k) x[k] = v[k] + r[k];
-    encode_chunk(x, n, h, e_out);
-    decode_chunk(h, n, *e_out, decoded);
-    for (int k = 0; k < n; ++k) {
-        float d = x[k] - decoded[k];
-        r[k] = std::isfinite(d) ? d : 0.f;
+/* Predictive scaled-FP16 strip codec, the definition the GPU strip store
+ * implements.  Per field and ring cell, slots in time order: the encoder
+ * codes the residual x = (v - p) + r of the value against p, the previous
+ * slot's decoded value, with r the previous slot's quantization error fed
+ * back; the chunk of 64 cells shares the exponent of max|x| (encode_chunk),
+ * d = decode(x) is x rounded to 11 bits, r' = x - d is exact, and the encoder
+ * moves to p' = p + d (a non-finite p' or r' restarts at zero).  The decoded
+ * sequence is therefore v^s + r^{s-1} - r^s.  Every kStripKey-th slot
+ * (s % kStripKey == kStripKey - 1) and the field's last slot keep p' in FP32
+ * as a keyframe.  Decoding runs backwards, as the reconstruction uses the
+ * strips: v^s = keyframe, or v^{s+1} - d^{s+1} in FP32. */
+constexpr int kStripKey = 32;
+
+struct Codec16 {
+    size_t ncells = 0, nch = 0;
+    int nfields = 0;
+    std::vector<int> last;      /* last slot stored, per field */
+    std::vector<uint16_t> h;    /* [slot][field][cell] residuals */
+    std::vector<int8_t> e;      /* [slot][field][chunk] */
+    std::vector<float> key;     /* [slot / kStripKey][field][cell] */
+    std::vector<float> p, r;    /* encoder state [field][cell] */
+    std::vector<float> dec;     /* [2][field][cell] decoded slots by parity */
+    std::vector<int> cur;       /* per field: lowest decoded slot (-1: none yet) */
+    void init(size_t cells, int slots, int fields, const std::vector<int> &lastslot) {
+        ncells = cells; nch = (cells + 63) / 64; nfields = fields; last = lastslot;
+        h.assign((size_t)slots * fields * cells, 0);
+        e.assign((size_t)slots * fields * nch, 0);
+        key.assign((size_t)((slots + kStripKey - 1) / kStripKey) * fields * cells, 0.f);
+        p.assign((size_t)fields * cells, 0.f);
+        r.assign((size_t)fields * cells, 0.f);
+        dec.assign((size_t)2 * fields * cells, 0.f);
+        cur.assign(fields, -1);
     }
-}
+    bool keyframe(int s, int f) const { return s % kStripKey == kStripKey - 1 || s == last[f]; }
+    void encode(int s, int f, const float *v) {
+        float *pf = p.data() + (size_t)f * ncells, *rf = r.data() + (size_t)f * ncells;
+        uint16_t *hs = h.data() + ((size_t)s * nfields + f) * ncells;
+        int8_t *es = e.data() + ((size_t)s * nfields + f) * nch;
+        float *ks = keyframe(s, f) ? key.data() + ((size_t)(s / kStripKey) * nfields + f) * ncells : nullptr;
+        float x[64], d[64];
+        for (size_t c0 = 0, ci = 0; c0 < ncells; c0 += 64, ++ci) {
+            int n = (int)std::min<size_t>(64, ncells - c0);
+            for (int k = 0; k < n; ++k) x[k] = (v[c0 + k] - pf[c0 + k]) + rf[c0 + k];
+            encode_chunk(x, n, hs + c0, es + ci);
+            decode_chunk(hs + c0, n, es[ci], d);
+            for (int k = 0; k < n; ++k) {
+                float rr = x[k] - d[k];
+                float p1 = pf[c0 + k] + d[k];
+                rf[c0 + k] = std::isfinite(rr) ? rr : 0.f;
+                pf[c0 + k] = std::isfinite(p1) ? p1 : 0.f;
+                if (ks) ks[c0 + k] = pf[c0 + k];
+            }
+        }
+    }
+    /* Decoded values of slot s of field f; uses must come in non-increasing
+     * slot order (s may be the last or the previous decoded slot). */
+    const float *decoded(int s, int f) {
+        float *d0 = dec.data() + (size_t)f * ncells;
+        auto buf = [&](int slot) { return d0 + (size_t)(slot & 1) * nfields * ncells; };
+        if (cur[f] < 0) {
+            const float *k = key.data() + ((size_t)(last[f] / kStripKey) * nfields + f) * ncells;
+            std::copy(k, k + ncells, buf(last[f]));
+            cur[f] = last[f];
+        }
+        while (cur[f] > s) {
+            const int t = cur[f] - 1;
+            float *o = buf(t);
+            if (keyframe(t, f)) {
+                const float *k = key.data() + ((size_t)(t / kStripKey) * nfields + f) * ncells;
+                std::copy(k, k + ncells, o);
+            } else {
+                const uint16_t *hs = h.data() + ((size_t)(t + 1) * nfields + f) * ncells;
+                const int8_t *es = e.data() + ((size_t)(t + 1) * nfields + f) * nch;
+                const float *up = buf(t + 1);
+                for (size_t q = 0; q < ncells; ++q)
+                    o[q] = up[q] - std::ldexp(f16_to_f32(hs[q]), (int)es[q >> 6]);
+            }
+            cur[f] = t;
+        }
+        return buf(s);
+    }
+};
 
 template <class R>
 struct Strips {
     Ring ring;
     int nslots = 0;
     bool fp16 = false;
-    std::vector<R> data; /* [slot][field][ring] */
-    std::vector<float> err; /* FP16 encoder error state [field][ring]; slots are saved in time order */
+    std::vector<R> data; /* FP32 mode: [slot][field][ring] */
+    Codec16 codec;       /* FP16 mode; slots are saved in time order, restored in reverse */
     void init(const Geo &g, int slots, bool use_fp16) {
         ring.init(g);
         nslots = slots;
         fp16 = use_fp16;
-        data.assign((size_t)slots * RS_NF * ring.ncells, (R)0);
-        err.assign(use_fp16 ? (size_t)RS_NF * ring.ncells : 0, 0.f);
+        if (fp16) codec.init(ring.ncells, slots, RS_NF, std::vector<int>(RS_NF, slots - 1));
+        else data.assign((size_t)slots * RS_NF * ring.ncells, (R)0);
     }
     R *slot(int t, int f) { return data.data() + ((size_t)t * RS_NF + f) * ring.ncells; }
     void save(int t, const Wave<R> &W) {
         const std::vector<R> *src[RS_NF] = {&W.vx, &W.vz, &W.th, &W.om, &W.uzx, &W.uzz, &W.uxx, &W.uxz};
+        std::vector<float> v(fp16 ? ring.ncells : 0);
         for (int f = 0; f < RS_NF; ++f) {
-            R *d = slot(t, f);
             for (size_t r = 0; r < ring.ncells; ++r) {
                 int i, j;
                 ring.cell(r, i, j);
-                d[r] = (*src[f])[(size_t)i * ring.Zn + j];
+                const R x = (*src[f])[(size_t)i * ring.Zn + j];
+                if (fp16) v[r] = (float)x;
+                else slot(t, f)[r] = x;
             }
-            if (fp16) {
-                /* quantize in place: the FP32 value goes through the codec */
-                float v[64], dec[64];
-                uint16_t h[64];
-                float *r = err.data() + (size_t)f * ring.ncells;
-                for (size_t c0 = 0; c0 < ring.ncells; c0 += 64) {
-                    int n = (int)std::min<size_t>(64, ring.ncells - c0);
-                    for (int k = 0; k < n; ++k) v[k] = (float)d[c0 + k];
-                    int8_t e;
-                    encode_chunk_fb(v, n, r + c0, h, &e, dec);
-                    for (int k = 0; k < n; ++k) d[c0 + k] = (R)dec[k];
-                }
-            }
+            if (fp16) codec.encode(t, f, v.data());
         }
     }
     void restore(int t, int f0, int f1, std::vector<R> *const *dst) {
         for (int f = f0; f < f1; ++f) {
-            const R *d = slot(t, f);
             std::vector<R> &out = *dst[f - f0];
+            const float *d16 = fp16 ? codec.decoded(t, f) : nullptr;
+            const R *d = fp16 ? nullptr : slot(t, f);
             for (size_t r = 0; r < ring.ncells; ++r) {
                 int i, j;
                 ring.cell(r, i, j);
-                out[(size_t)i * ring.Zn + j] = d[r];
+                out[(size_t)i * ring.Zn + j] = fp16 ? (R)d16[r] : d[r];
             }
         }
     }
@@ -790,18 +847,22 @@ int oracle_ring(const ouc_params *p, int *ij, int64_t *ncells) {
     return OUC_OK;
 }
 
-/* Scaled-FP16 strip codec: `steps` slots of one field, x as [steps][n], in
- * chunks of 64 with the error feedback carried from slot to slot (zero at the
- * first).  h is [steps][n], e is [steps][ceil(n/64)]. */
-int oracle_fp16_encode(const float *x, int64_t n, int32_t steps, uint16_t *h, int8_t *e) {
+/* Strip codec: `steps` slots of one field, x as [steps][n], coded in order
+ * with the predictive FP16 codec (the last slot is a keyframe).  h is
+ * [steps][n] residual bits, e is [steps][ceil(n/64)] exponents; decoded
+ * (optional) is [steps][n], decoded backwards as the reconstruction does. */
+int oracle_fp16_encode(const float *x, int64_t n, int32_t steps, uint16_t *h, int8_t *e, float *decoded) {
     if (!x || n < 1 || steps < 1 || !h || !e) return OUC_INVALID_ARG;
     const int64_t nch = (n + 63) / 64;
-    std::vector<float> r((size_t)n, 0.f);
-    float dec[64];
-    for (int32_t t = 0; t < steps; ++t)
-        for (int64_t c0 = 0, ci = 0; c0 < n; c0 += 64, ++ci) {
-            int k = (int)std::min<int64_t>(64, n - c0);
-            encode_chunk_fb(x + (size_t)t * n + c0, k, r.data() + c0, h + (size_t)t * n + c0, e + t * nch + ci, dec);
+    Codec16 c;
+    c.init((size_t)n, steps, 1, std::vector<int>(1, steps - 1));
+    for (int32_t t = 0; t < steps; ++t) c.encode(t, 0, x + (size_t)t * n);
+    std::copy(c.h.begin(), c.h.end(), h);
+    std::copy(c.e.begin(), c.e.begin() + (size_t)steps * nch, e);
+    if (decoded)
+        for (int32_t t = steps - 1; t >= 0; --t) {
+            const float *d = c.decoded(t, 0);
+            std::copy(d, d + n, decoded + (size_t)t * n);
         }
     return OUC_OK;
 }

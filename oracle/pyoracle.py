@@ -36,7 +36,7 @@ def lib():
         L.oracle_dop.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_double, dp]
         L.oracle_mute_thresholds.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), dp]
         L.oracle_ring.argtypes = [P(abi.Params), P(C.c_int), P(C.c_int64)]
-        L.oracle_fp16_encode.argtypes = [P(C.c_float), C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8)]
+        L.oracle_fp16_encode.argtypes = [P(C.c_float), C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8), P(C.c_float)]
         L.oracle_fp16_decode.argtypes = [P(C.c_uint16), P(C.c_int8), C.c_int64, P(C.c_float)]
         L.oracle_f32_to_f16.argtypes = [P(C.c_float), C.c_int64, P(C.c_uint16)]
         L.oracle_forward_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
@@ -116,17 +116,21 @@ def ring(p):
     return ij
 
 
-def fp16_encode(x):
+def fp16_encode(x, decoded=False):
     """Strip encoder: 1-D x is one slot; 2-D x [steps][n] is successive slots
-    of one field with the error feedback carried between them."""
+    of one field, coded predictively with error feedback.  Returns (residual
+    bits, chunk exponents) and, with decoded=True, the values decoded
+    backwards as the reconstruction does."""
     x = np.ascontiguousarray(x, np.float32)
     x2 = x.reshape(1, -1) if x.ndim == 1 else x
     steps, n = x2.shape
     h = np.zeros((steps, n), np.uint16)
     e = np.zeros((steps, (n + 63) // 64), np.int8)
+    d = np.zeros((steps, n), np.float32)
     _check(lib().oracle_fp16_encode(abi.fptr(x2), n, steps, h.ctypes.data_as(C.POINTER(C.c_uint16)),
-                                    e.ctypes.data_as(C.POINTER(C.c_int8))), "fp16_encode")
-    return (h[0], e[0]) if x.ndim == 1 else (h, e)
+                                    e.ctypes.data_as(C.POINTER(C.c_int8)), abi.fptr(d)), "fp16_encode")
+    out = (h, e, d) if decoded else (h, e)
+    return tuple(a[0] for a in out) if x.ndim == 1 else out
 
 
 def fp16_decode(h, e):

@@ -83,10 +83,13 @@ ouc::KStrips ouc_ctx::kstrips() const {
     s.f32 = strips32.f();
     s.f16 = strips16.as<__half>();
     s.e8 = stripse8.as<int8_t>();
+    s.key = stripskey.f();
     s.err = stripserr.f();
+    s.dec = stripsdec.f();
     s.ncells = ring.ncells;
     s.nchunks = strip_chunks;
     s.fp16 = p.strip_precision == OUC_STRIP_FP16;
+    s.last_v = s.last_th = (int)strip_slots - 1; // slots 0..nt-2 of both field groups
     return s;
 }
 
@@ -94,12 +97,15 @@ void ouc_ctx::ensure_strips() {
     int want16 = p.strip_precision == OUC_STRIP_FP16;
     int64_t slots = g.nt - 1;
     if (strip_fp16 == want16 && strip_slots == slots) return;
-    strips32.release(); strips16.release(); stripse8.release(); stripserr.release();
+    strips32.release(); strips16.release(); stripse8.release(); stripskey.release(); stripserr.release();
+    stripsdec.release();
     strip_chunks = (ring.ncells + 63) / 64;
     if (want16) {
         strips16.alloc((size_t)slots * RS_NF * ring.ncells * sizeof(__half));
         stripse8.alloc((size_t)slots * RS_NF * strip_chunks);
-        stripserr.alloc((size_t)RS_NF * ring.ncells * sizeof(float));
+        stripskey.alloc((size_t)((slots + kStripKey - 1) / kStripKey) * RS_NF * ring.ncells * sizeof(float));
+        stripserr.alloc((size_t)2 * RS_NF * ring.ncells * sizeof(float));
+        stripsdec.alloc((size_t)2 * RS_NF * ring.ncells * sizeof(float));
     } else {
         strips32.alloc((size_t)slots * RS_NF * ring.ncells * sizeof(float));
     }
@@ -281,8 +287,13 @@ void run_backward_plain(ouc_ctx *c, const ouc_shot *s, const std::vector<float> 
                                                             c->vp3[(nt - 1) % 3][1].f()));
     const float *recx = c->rec.f();
     const float *recz = c->rec.f() + (size_t)nt * g.nx;
+    // FP16 strips decode backwards one slot per step: step t restores slots t
+    // and (EXACT) t-1
+    const dim3 dgrid(rblocks, RS_NF, 1);
+    if (ks.fp16) k_strip_decode<<<dgrid, 256, 0, st>>>(ks, nt - 2, KStripStride{});
     for (int t = nt - 1; t >= 0; --t) {
         if (t < nt - 1) {
+            if (ks.fp16 && t >= 1) k_strip_decode<<<dgrid, 256, 0, st>>>(ks, t - 1, KStripStride{});
             OUC_DISPATCH_N(N, k_recon_velocity<NN><<<frame, kTile, 0, st>>>(kg, fw, km));
             int sv = exact ? t - 1 : t;
             k_strip_restore_v<<<rblocks, 256, 0, st>>>(kr, ks, sv, fw.Vx, fw.Vz, g.P);
@@ -559,29 +570,44 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, floa
 }
 
 ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, int32_t steps, uint16_t *half_bits,
-                                int8_t *exponents) {
+                                int8_t *exponents, float *decoded) {
     if (!x || n < 1 || steps < 1 || !half_bits || !exponents) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
     try {
         OUC_HIP(hipSetDevice(device));
         int64_t chunks = (n + 63) / 64;
-        DevBuf din, dh, de, derr;
+        const size_t per = (size_t)RS_NF * n; // one slot of the store's layout; field 0 is used
+        DevBuf din, dh, de, dkey, derr, ddec, dout;
         din.alloc((size_t)steps * n * sizeof(float));
-        dh.alloc((size_t)steps * n * sizeof(__half));
-        de.alloc((size_t)steps * chunks);
-        derr.alloc(n * sizeof(float));
+        dh.alloc((size_t)steps * per * sizeof(__half));
+        de.alloc((size_t)steps * RS_NF * chunks);
+        dkey.alloc((size_t)((steps + kStripKey - 1) / kStripKey) * per * sizeof(float));
+        derr.alloc(2 * per * sizeof(float));
+        ddec.alloc(2 * per * sizeof(float));
+        dout.alloc((size_t)steps * n * sizeof(float));
         OUC_HIP(hipMemcpy(din.ptr, x, (size_t)steps * n * sizeof(float), hipMemcpyHostToDevice));
         OUC_HIP(hipMemset(derr.ptr, 0, derr.bytes));
         KStrips s{};
         s.f16 = dh.as<__half>();
         s.e8 = de.as<int8_t>();
+        s.key = dkey.f();
         s.err = derr.f();
+        s.dec = ddec.f();
         s.ncells = n;
         s.nchunks = chunks;
         s.fp16 = 1;
-        k_codec_fp16<<<(unsigned)((n + 255) / 256), 256>>>(s, din.f(), n, steps);
+        s.last_v = s.last_th = steps - 1;
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        k_codec_fp16<<<blocks, 256>>>(s, din.f(), n, steps);
+        for (int t = steps - 1; t >= 0; --t) { // backward decoding, as the reconstruction does
+            k_strip_decode<<<dim3(blocks, 1, 1), 256>>>(s, t, KStripStride{});
+            OUC_HIP(hipMemcpy(dout.f() + (size_t)t * n, ddec.f() + (size_t)(t & 1) * per, n * sizeof(float),
+                              hipMemcpyDeviceToDevice));
+        }
         OUC_HIP(hipGetLastError());
-        OUC_HIP(hipMemcpy(half_bits, dh.ptr, (size_t)steps * n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-        OUC_HIP(hipMemcpy(exponents, de.ptr, (size_t)steps * chunks, hipMemcpyDeviceToHost));
+        OUC_HIP(hipMemcpy2D(half_bits, n * sizeof(uint16_t), dh.ptr, per * sizeof(__half), n * sizeof(uint16_t), steps,
+                            hipMemcpyDeviceToHost));
+        OUC_HIP(hipMemcpy2D(exponents, chunks, de.ptr, (size_t)RS_NF * chunks, chunks, steps, hipMemcpyDeviceToHost));
+        if (decoded) OUC_HIP(hipMemcpy(decoded, dout.ptr, (size_t)steps * n * sizeof(float), hipMemcpyDeviceToHost));
         return OUC_OK;
     } catch (const Error &e) {
         return fail(nullptr, e.code, e.what());

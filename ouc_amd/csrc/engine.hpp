@@ -77,7 +77,8 @@ struct ouc_ctx {
     ouc::DevBuf dX, dZ; // Xn
     ouc::DevBuf counts; // nx
     ouc::DevBuf img[4]; // pp, ps, pp_lap, ps_lap (nx*nz)
-    ouc::DevBuf strips32, strips16, stripse8, stripserr; // stripserr: FP16 encoder error state [RS_NF][ring]
+    ouc::DevBuf strips32, strips16, stripse8; // FP32 strips, or FP16 residuals and chunk exponents
+    ouc::DevBuf stripskey, stripserr, stripsdec; // FP16: keyframes, encoder state [2][RS_NF][ring], decoded [2][RS_NF][ring]
     int64_t strip_slots = 0, strip_chunks = 0;
     int strip_fp16 = -1;
     hipEvent_t ev[8] = {};

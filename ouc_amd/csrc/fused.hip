@@ -73,8 +73,9 @@ struct FusedState {
     FWave S{}, R{};
     DevBuf vp[3][2];       // source P velocity levels [cap][plane]
     DevBuf acc[3];         // npp, nps, den
-    DevBuf strips32, strips16, e8, ef; // ef: FP16 encoder error state [cap][RS_NF][ring]
-    long shot_strip = 0, shot_chunks = 0, nchunks = 0;
+    DevBuf strips32, strips16, e8;   // FP32 strips, or FP16 residuals and chunk exponents
+    DevBuf keyb, ef, decb;           // FP16: keyframes, encoder state, decoded slots ([cap][2][RS_NF][ring])
+    long shot_strip = 0, shot_chunks = 0, shot_key = 0, shot_aux = 0, nchunks = 0;
     int nslots = 0, fp16 = 0;
     DevBuf rec;            // [cap][2][nt][nx]
     DevBuf srcpos, wav, counts, t0;
@@ -257,7 +258,10 @@ size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
     size_t b = 2 * kWaveArrays * plane + 6 * plane + 3 * plane;
     const size_t cells = (size_t)c->ring.ncells;
     b += (size_t)g.nt * RS_NF * cells * (fp16 ? sizeof(__half) : sizeof(float));
-    if (fp16) b += (size_t)g.nt * RS_NF * ((cells + 63) / 64) + (size_t)RS_NF * cells * sizeof(float);
+    if (fp16)
+        b += (size_t)g.nt * RS_NF * ((cells + 63) / 64) +
+             (size_t)((g.nt + kStripKey - 1) / kStripKey) * RS_NF * cells * sizeof(float) +
+             (size_t)4 * RS_NF * cells * sizeof(float);
     b += (size_t)2 * g.nt * g.nx * sizeof(float) + (size_t)g.nt * sizeof(float) + g.nx * sizeof(int32_t);
     return b;
 }
@@ -284,11 +288,16 @@ void ensure(ouc_ctx *c, int shots) {
     st->nchunks = (c->ring.ncells + 63) / 64;
     st->shot_strip = (long)st->nslots * RS_NF * c->ring.ncells;
     st->shot_chunks = (long)st->nslots * RS_NF * st->nchunks;
-    st->strips32.release(); st->strips16.release(); st->e8.release(); st->ef.release();
+    st->shot_key = (long)((st->nslots + kStripKey - 1) / kStripKey) * RS_NF * c->ring.ncells;
+    st->shot_aux = (long)2 * RS_NF * c->ring.ncells;
+    st->strips32.release(); st->strips16.release(); st->e8.release();
+    st->keyb.release(); st->ef.release(); st->decb.release();
     if (want16) {
         st->strips16.alloc((size_t)st->shot_strip * shots * sizeof(__half));
         st->e8.alloc((size_t)st->shot_chunks * shots);
-        st->ef.alloc((size_t)RS_NF * c->ring.ncells * shots * sizeof(float));
+        st->keyb.alloc((size_t)st->shot_key * shots * sizeof(float));
+        st->ef.alloc((size_t)st->shot_aux * shots * sizeof(float));
+        st->decb.alloc((size_t)st->shot_aux * shots * sizeof(float));
     } else {
         st->strips32.alloc((size_t)st->shot_strip * shots * sizeof(float));
     }
@@ -369,7 +378,11 @@ KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
     s.f32 = st->strips32.f();
     s.f16 = st->strips16.as<__half>();
     s.e8 = st->e8.as<int8_t>();
+    s.key = st->keyb.f();
     s.err = st->ef.f();
+    s.dec = st->decb.f();
+    s.last_v = st->nslots - 2;  // velocity totals V^{t+1} are saved for t < nt-1
+    s.last_th = st->nslots - 1; // the stress-like fields for every t
     s.ncells = c->ring.ncells;
     s.nchunks = st->nchunks;
     s.fp16 = st->fp16;
@@ -544,6 +557,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     unsigned int *mon_f = st->mon.as<unsigned int>(), *mon_r = mon_f + 2 * st->cap;
     const KRing kr = c->kring();
     const KStrips ks0 = base_strips(st, c);
+    const KStripStride sstride{st->shot_strip, st->shot_chunks, st->shot_key, st->shot_aux};
     const int rblocks = (int)((c->ring.ncells + 255) / 256);
     int nlaunch = 0;
 
@@ -575,11 +589,9 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             FWave ws = st->S; // the stress memory variables of this step (the streaming band kernel double-buffers them)
             if (st->edge && rb == 0)
                 for (int q = 0; q < 4; ++q) ws.F[q] = st->S.F2[q];
-            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, ws, kr, ks0, st->shot_strip, st->shot_chunks, t, 0,
-                                                               0);
+            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, ws, kr, ks0, sstride, t, 0, 0);
             if (t < nt - 1)
-                k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, st->shot_strip, st->shot_chunks,
-                                                                   t, 1 - rb, 1);
+                k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, sstride, t, 1 - rb, 1);
         }
         rb = 1 - rb;
     }
@@ -621,20 +633,24 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     // So the two run on two streams, joined at the end of every iteration.
     // Eager launches (profiling) stay on one stream.
     hipStream_t sr = graph_b ? st->s2 : s;
+    // FP16 strips decode backwards one slot per step, on the reconstruction's
+    // stream: iteration t decodes slot t, and the reconstruction of step t reads
+    // slots t+1 and (EXACT) t
+    const dim3 dgrid(rblocks, RS_NF, B);
     run_loop(s, graph_b, st->gbwd, B, rtm, ncomp, [&] {
     for (int t = nt - 1; t >= 0; --t) {
         if (graph_b) {
             OUC_HIP(hipEventRecord(st->evf, s));
             OUC_HIP(hipStreamWaitEvent(sr, st->evf, 0));
         }
+        if (ks0.fp16) k_strip_decode<<<dgrid, 256, 0, sr>>>(ks0, t, sstride); // slot t (t = nt-1: the first)
         if (t < nt - 1) {
             // theta^{t+1} from V^{t+1}: ring totals from slot t+1 (FAITHFUL, one step late as the
             // reference) or slot t (EXACT); the hand-off step uses the forward's own state
             FRecon rc{};
             rc.ring = kr;
             rc.strips = ks0;
-            rc.shot_strip = st->shot_strip;
-            rc.shot_chunks = st->shot_chunks;
+            rc.stride = sstride;
             const bool handoff = t == nt - 2;
             rc.slot_v = handoff ? -1 : (exact ? t : t + 1);
             rc.slot_th = t + 1;
@@ -702,7 +718,7 @@ int default_capacity(ouc_ctx *c, int nshots) {
     // keep what this context already holds available for reuse
     FusedState *st = fs(c);
     size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes +
-                  st->ef.bytes;
+                  st->keyb.bytes + st->ef.bytes + st->decb.bytes;
     size_t avail = (size_t)((freeb + held) * 0.85);
     size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
     int cap = (int)std::max<size_t>(1, avail / per);

@@ -154,8 +154,7 @@ __device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k
 struct FRecon {
     KRing ring;
     KStrips strips;      // base for shot 0; per-shot offset below
-    long shot_strip;     // elements between shots' strip stores
-    long shot_chunks;    // exponent entries between shots
+    KStripStride stride; // between shots' strip stores
     int slot_v;          // slot of the ring totals (< 0: the state's own, no override)
     int slot_th;         // slot of the ring stress-like fields
     int exact;           // add the source term (EXACT semantics, or the hand-off step)
@@ -173,14 +172,7 @@ __device__ __forceinline__ long ring_index(const KRing &r, int i, int j) {
     return 2 * r.ntop + r.nside + (long)(i - (r.Xn - L - N)) * w + (j - (L + N));
 }
 
-__device__ __forceinline__ KStrips shot_strips(const FRecon &rc) {
-    KStrips s = rc.strips;
-    const long so = (long)blockIdx.z * rc.shot_strip, eo = (long)blockIdx.z * rc.shot_chunks;
-    if (s.f32) s.f32 += so;
-    if (s.f16) s.f16 += so;
-    if (s.e8) s.e8 += eo;
-    return s;
-}
+__device__ __forceinline__ KStrips shot_strips(const FRecon &rc) { return strips_of_shot(rc.strips, rc.stride, blockIdx.z); }
 
 // Totals seen by the reconstruction's stress pass at (i, j).
 __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s, int rb,
@@ -740,18 +732,14 @@ __global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, cons
 }
 
 // Strip store from the stored band stress and the written V (forward).
-__global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, long shot_strip,
-                                                     long shot_chunks, int slot, int vbuf, int with_v) {
+__global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, KStripStride d, int slot,
+                                                     int vbuf, int with_v) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = q < r.ncells;
     int i = r.L, j = r.L;
     if (valid) ring_cell(r, q, i, j);
-    KStrips s = s0;
+    const KStrips s = strips_of_shot(s0, d, blockIdx.z);
     const long so = (long)blockIdx.z * g.plane;
-    if (s.f32) s.f32 += (long)blockIdx.z * shot_strip;
-    if (s.f16) s.f16 += (long)blockIdx.z * shot_strip;
-    if (s.e8) s.e8 += (long)blockIdx.z * shot_chunks;
-    if (s.err) s.err += (long)blockIdx.z * RS_NF * r.ncells;
     const long k = so + (long)i * g.P + j;
     if (with_v) {
         strip_put(s, slot, RS_VX, q, valid, valid ? w.V[vbuf][0][k] : 0.f);

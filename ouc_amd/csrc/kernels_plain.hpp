@@ -302,23 +302,53 @@ __device__ __forceinline__ void ring_cell(const KRing &r, long q, int &i, int &j
 // four gradients g' = g - F (the reference's du / vs^2) of step t.
 enum { RS_VX = 0, RS_VZ, RS_TH, RS_OM, RS_GZX, RS_GZZ, RS_GXX, RS_GXZ, RS_NF };
 
-// FP32 and scaled-FP16 strip stores.  FP16: each 64-value chunk of one field
-// shares a power-of-two scale 2^e with max|x| * 2^-e in [2^14, 2^15).
+// FP32 and scaled-FP16 strip stores.
 //
-// The FP16 encoder uses error feedback: it keeps, per field and ring cell, the
-// quantization error r of the previous slot and codes x = v + r instead of v,
-// then keeps r' = x - decode(x).  The decoded sequence is v^t + r^{t-1} - r^t,
-// so its error is the first time difference of a bounded sequence (noise
-// shaped away from the low frequencies the wavefield occupies).  Slots must
-// be stored in time order; `err` is zero before the first.
+// FP16 strips are predictive.  Slot s of a field stores, per ring cell, the
+// residual x = (v^s - p) + r of the value against p, the previous slot's
+// decoded value, with r the previous slot's quantization error fed back
+// (error feedback).  Each 64-value chunk of a slot shares a power-of-two scale
+// 2^e with max|x| * 2^-e in [2^14, 2^15); the residual is stored as binary16
+// of x * 2^-e, so d = decode(x) is x rounded to 11 bits and x - d is exact.
+// The encoder then holds p' = p + d and r' = x - d: its decoded sequence is
+// v^s + r^{s-1} - r^s, an error that is a first time difference (small at the
+// wavefield's frequencies) of a quantity bounded by half a unit of the
+// residual, itself about the one-step change of v.  Every kStripKey-th slot,
+// and the field's last slot, also keeps p' in FP32 as a keyframe.  The
+// reconstruction decodes backwards one slot per step, v^s = keyframe or
+// v^{s+1} - d^{s+1}, into a two-slot buffer; the FP32 rounding of that
+// subtraction is bounded by the keyframe spacing.  Slots are stored in time
+// order, with the encoder state zero before the first.
+constexpr int kStripKey = 32;
 struct KStrips {
-    float *f32;          // [slot][RS_NF][ncells]
-    __half *f16;         // [slot][RS_NF][ncells]
-    int8_t *e8;          // [slot][RS_NF][nchunks]
-    float *err;          // [RS_NF][ncells] encoder error state (FP16 only)
+    float *f32;          // [slot][RS_NF][ncells] (FP32 strips)
+    __half *f16;         // [slot][RS_NF][ncells] residuals (FP16 strips)
+    int8_t *e8;          // [slot][RS_NF][nchunks] chunk exponents
+    float *key;          // [slot / kStripKey][RS_NF][ncells] keyframes
+    float *err;          // [2][RS_NF][ncells] encoder state: previous decoded value, fed-back error
+    float *dec;          // [2][RS_NF][ncells] decoded slots, by slot parity
     long ncells, nchunks;
     int fp16;
+    int last_v, last_th; // last slot stored for the two velocity fields / the six stress-like fields
 };
+
+// Element strides between the strip stores of consecutive shots of a batch.
+struct KStripStride {
+    long strip;  // f32 / f16 elements
+    long chunks; // exponent entries
+    long key;    // keyframe elements
+    long aux;    // encoder-state and decoded-slot elements
+};
+
+__device__ __forceinline__ KStrips strips_of_shot(KStrips s, const KStripStride &d, long z) {
+    if (s.f32) s.f32 += z * d.strip;
+    if (s.f16) s.f16 += z * d.strip;
+    if (s.e8) s.e8 += z * d.chunks;
+    if (s.key) s.key += z * d.key;
+    if (s.err) s.err += z * d.aux;
+    if (s.dec) s.dec += z * d.aux;
+    return s;
+}
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -334,6 +364,10 @@ __device__ __forceinline__ int scale_exponent(float mx) {
     return e < -127 ? -127 : (e > 127 ? 127 : e);
 }
 
+__device__ __forceinline__ bool strip_keyframe(const KStrips &s, int slot, int f) {
+    return slot % kStripKey == kStripKey - 1 || slot == (f < RS_TH ? s.last_v : s.last_th);
+}
+
 // Store `v` of ring cell q (one lane per cell, 64 consecutive cells per wave).
 __device__ __forceinline__ void strip_put(const KStrips &s, int slot, int f, long q, bool valid, float v) {
     const long base = ((long)slot * RS_NF + f) * s.ncells;
@@ -341,38 +375,66 @@ __device__ __forceinline__ void strip_put(const KStrips &s, int slot, int f, lon
         if (valid) s.f32[base + q] = v;
         return;
     }
-    float *ep = (valid && s.err) ? s.err + (long)f * s.ncells + q : nullptr;
-    const float x = ep ? v + *ep : v;
+    const long fc = (long)f * s.ncells + q;
+    float *const pp = s.err + fc, *const rp = s.err + (long)RS_NF * s.ncells + fc;
+    const float pred = valid ? *pp : 0.f, fb = valid ? *rp : 0.f;
+    const float x = (v - pred) + fb;
     float mx = wave_max(valid ? fabsf(x) : 0.f);
     int e = scale_exponent(mx);
     const __half h = __float2half_rn(ldexpf(x, -e));
-    if (valid) s.f16[base + q] = h;
-    if (ep) {
-        const float r = x - ldexpf(__half2float(h), e); // exact: h is x rounded to 11 bits
-        *ep = isfinite(r) ? r : 0.f;
+    if (valid) {
+        s.f16[base + q] = h;
+        const float d = ldexpf(__half2float(h), e);
+        const float r = x - d; // exact: d is x rounded to 11 bits
+        float p1 = pred + d;
+        p1 = isfinite(p1) ? p1 : 0.f;
+        *pp = p1;
+        *rp = isfinite(r) ? r : 0.f;
+        if (strip_keyframe(s, slot, f)) s.key[((long)(slot / kStripKey) * RS_NF + f) * s.ncells + q] = p1;
+        if ((q & 63) == 0) s.e8[((long)slot * RS_NF + f) * s.nchunks + (q >> 6)] = (int8_t)e;
     }
-    if (valid && (q & 63) == 0) s.e8[((long)slot * RS_NF + f) * s.nchunks + (q >> 6)] = (int8_t)e;
 }
 
+// FP16: `slot` must be one of the last two slots decoded by k_strip_decode.
 __device__ __forceinline__ float strip_get(const KStrips &s, int slot, int f, long q) {
-    const long base = ((long)slot * RS_NF + f) * s.ncells;
-    if (!s.fp16) return s.f32[base + q];
-    int e = s.e8[((long)slot * RS_NF + f) * s.nchunks + (q >> 6)];
-    return ldexpf(__half2float(s.f16[base + q]), e);
+    if (!s.fp16) return s.f32[((long)slot * RS_NF + f) * s.ncells + q];
+    return s.dec[((long)(slot & 1) * RS_NF + f) * s.ncells + q];
 }
 
-// The strip codec alone: `steps` consecutive slots of one field, x and the
-// outputs as [steps][n] (s.err, when set, carries the feedback across them).
+// Backward decoding of slot `slot` of every field (blockIdx.y) into
+// dec[slot & 1]: the keyframe, or slot+1's decoded value minus slot+1's
+// residual (slot+1 must be the slot decoded before).  blockIdx.z is the shot,
+// with the given element strides between shots.
+static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slot, KStripStride d) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (q >= s0.ncells) return;
+    const KStrips s = strips_of_shot(s0, d, blockIdx.z);
+    const __half *f16 = s.f16;
+    const int8_t *e8 = s.e8;
+    const float *key = s.key;
+    float *dec = s.dec;
+    const int last = f < RS_TH ? s.last_v : s.last_th;
+    float v = 0.f;
+    if (slot >= 0 && slot <= last) {
+        if (strip_keyframe(s, slot, f)) {
+            v = key[((long)(slot / kStripKey) * RS_NF + f) * s.ncells + q];
+        } else {
+            const long b1 = (long)(slot + 1) * RS_NF + f;
+            const float d = ldexpf(__half2float(f16[b1 * s.ncells + q]), e8[b1 * s.nchunks + (q >> 6)]);
+            v = dec[((long)((slot + 1) & 1) * RS_NF + f) * s.ncells + q] - d;
+        }
+    }
+    dec[((long)(slot & 1) * RS_NF + f) * s.ncells + q] = v;
+}
+
+// The strip encoder alone: `steps` consecutive slots of field 0, x as
+// [steps][n]; the outputs keep the store's layout (s.last_v = steps - 1).
 static __global__ __launch_bounds__(256) void k_codec_fp16(KStrips s, const float *__restrict__ x, long n,
                                                            int steps) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = q < n;
-    for (int t = 0; t < steps; ++t) {
-        KStrips st = s;
-        st.f16 = s.f16 + (long)t * n;
-        st.e8 = s.e8 + (long)t * s.nchunks;
-        strip_put(st, 0, 0, q, valid, valid ? x[(long)t * n + q] : 0.f);
-    }
+    for (int t = 0; t < steps; ++t) strip_put(s, t, 0, q, valid, valid ? x[(long)t * n + q] : 0.f);
 }
 
 // Velocity part of slot `slot` from the assembled totals.  Launched with

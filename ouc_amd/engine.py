@@ -47,7 +47,7 @@ def lib():
     L.ouc_set_profiling.restype = C.c_int
     L.ouc_write_image_files.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, fp, fp]
     L.ouc_write_image_files.restype = C.c_int
-    L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8)]
+    L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8), fp]
     L.ouc_strip_codec_fp16.restype = C.c_int
     L.ouc_set_batch_capacity.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
     L.ouc_set_batch_capacity.restype = C.c_int
@@ -264,21 +264,24 @@ def write_image_files(directory, pp_lap, ps_lap, shot):
         raise OucError(st, lib().ouc_last_error(None).decode())
 
 
-def strip_codec_fp16(x, device=0):
+def strip_codec_fp16(x, device=0, decoded=False):
     """GPU scaled-FP16 strip codec on a host array.  A 1-D x is one slot; a
-    2-D x [steps][n] is `steps` slots of one field stored in order, with the
-    encoder's error feedback carried between them.  Returns (binary16 bits,
-    chunk exponents) shaped like x and [steps][ceil(n/64)]."""
+    2-D x [steps][n] is `steps` slots of one field stored in order (predictive
+    coding with error feedback).  Returns (binary16 residual bits, chunk
+    exponents) shaped like x and [steps][ceil(n/64)], and with decoded=True
+    the values decoded backwards as the reconstruction does."""
     x = np.ascontiguousarray(x, np.float32)
     x2 = x.reshape(1, -1) if x.ndim == 1 else x
     steps, n = x2.shape
     h = np.zeros((steps, n), np.uint16)
     e = np.zeros((steps, (n + 63) // 64), np.int8)
+    d = np.zeros((steps, n), np.float32)
     st = lib().ouc_strip_codec_fp16(int(device), abi.fptr(x2), n, steps, h.ctypes.data_as(C.POINTER(C.c_uint16)),
-                                    e.ctypes.data_as(C.POINTER(C.c_int8)))
+                                    e.ctypes.data_as(C.POINTER(C.c_int8)), abi.fptr(d))
     if st != abi.OUC_OK:
         raise OucError(st, lib().ouc_last_error(None).decode())
-    return (h[0], e[0]) if x.ndim == 1 else (h, e)
+    out = (h, e, d) if decoded else (h, e)
+    return tuple(a[0] for a in out) if x.ndim == 1 else out
 
 
 def _lb_check(st):
