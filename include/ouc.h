@@ -217,6 +217,34 @@ ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best
 /* gfx architecture name of a HIP device (e.g. "gfx950"), for tuning caches. */
 ouc_status ouc_device_arch(int32_t device, char *buf, int32_t len);
 
+/* Elastic FWI gradient of a set of shots, summed over the shots (fused engine,
+ * CPML boundary, every shot at the context's nt).  Replaces the gradient pass
+ * the reference declares but never launches: the adjoint residual
+ * (adsource, P-S-seperating.cu:1484-1498: modelled minus observed on every
+ * receiver and step, no mute) drives the receiver wavefield, and gradfun
+ * (P-S-seperating.cu:1518-1575) accumulates at every backward step t
+ *   grad_vp -= 2 vp (div u) theta_r
+ *   grad_vs -= 2 vs (omega_r (duxdz - duzdx) + duzdx_r duzdx + duzdz_r duzdz
+ *                    + duxdz_r duxdz + duxdx_r duxdx)
+ *   illum   += ux^2 + uz^2
+ * with u the reconstructed source velocity at level t+1 (the level the EXACT
+ * imaging condition uses) and the receiver stress fields of step t.  The
+ * derivatives of u use the engine's stencil: the reference's hard-coded
+ * 12th-order duxdz pairs f[j+4] with f[j-6] (P-S-seperating.cu:1556), which is
+ * not carried over.  The live physics has no density, so the reference's factor
+ * 2 P vp is 2 vp.  observed: [nshots][2][nx][nt] (vx, vz), receiver-major;
+ * grad_vp, grad_vs, illum: interior nx*nz x-major, any may be NULL; misfit
+ * (may be NULL): 0.5 * sum of squared residuals over all shots. */
+ouc_status ouc_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, const float *observed,
+                            float *grad_vp, float *grad_vs, float *illum, double *misfit);
+
+/* Model update of one array: out = v - step * 0.95^iter * grad, evaluated in
+ * double and rounded to float, as revise_model_end (P-S-seperating.cu:1500-1517,
+ * step a_2 = 30 at P-S-seperating.cu:25) does for Vp and Vs on the interior.
+ * Host arrays of n floats; device memory is held for the call only. */
+ouc_status ouc_model_update(int32_t device, int64_t n, const float *v, const float *grad, double step, int32_t iter,
+                            float *out);
+
 /* Timing of the last forward/RTM/batch call on this context. */
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);
 

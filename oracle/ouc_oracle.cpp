@@ -598,11 +598,42 @@ void copy_wave_stress(const Wave<R> &a, Wave<R> &b) {
     b.uxx = a.uxx; b.uxz = a.uxz; b.uzx = a.uzx; b.uzz = a.uzz;
 }
 
+/* gradfun (P-S-seperating.cu:1518-1575), one backward step on the interior:
+ * derivatives of the source velocities ux, uz with the engine's staggered
+ * operators, against the receiver's stress fields of the same step.  The
+ * reference hard-codes N = 6 and, in its duxdz sum, pairs f[j+4] with f[j-6]
+ * (P-S-seperating.cu:1556); here duxdz is the plain Dp_z of the stress pass.
+ * The live physics has no density (P = 1). */
+template <class R>
+void gradfun_step(const Setup<R> &S, const R *ux, const R *uz, const R *th, const R *om, const R *uxx, const R *uxz,
+                  const R *uzx, const R *uzz, R *gvp, R *gvs, R *ill) {
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn, L = g.L;
+#pragma omp parallel for schedule(static)
+    for (int i = L; i < g.Xn - L; ++i)
+        for (int j = L; j < Zn - L; ++j) {
+            size_t k = (size_t)i * Zn + j;
+            R dxx = dmx(ux, k, Zn, S.c, N, S.dx);
+            R dzz = dmz(uz, k, S.c, N, S.dz);
+            R dzx = dpx(uz, k, Zn, S.c, N, S.dx);
+            R dxz = dpz(ux, k, S.c, N, S.dz);
+            R vp = S.vp[k], vs = S.vs[k];
+            gvp[k] -= 2 * vp * ((dxx + dzz) * th[k]);
+            gvs[k] -= 2 * vs * (om[k] * (dxz - dzx) + uzx[k] * dzx + uzz[k] * dzz + uxz[k] * dxz + uxx[k] * dxx);
+            ill[k] += ux[k] * ux[k] + uz[k] * uz[k];
+        }
+}
+
 struct Outputs {
     double *seis = nullptr;      /* [6][nx][nt] */
     double *rec_out = nullptr;   /* [2][nx][nt] records used for RTM (after the mute) */
     double *img_pp = nullptr, *img_ps = nullptr, *lap_pp = nullptr, *lap_ps = nullptr; /* nx*nz */
     double *recon_err = nullptr; /* [nt]: max |V_recon - V_fwd| / max |V_fwd| per step */
+    /* FWI: observed records [2][nx][nt]; the receiver wavefield is driven by
+     * the residual (modelled - observed, no mute) and the gradient of
+     * gradfun accumulates over every backward step into nx*nz arrays */
+    const double *observed = nullptr;
+    double *grad_vp = nullptr, *grad_vs = nullptr, *illum = nullptr, *misfit = nullptr;
 };
 
 /* The whole shot: forward loop (P-S-seperating.cu:2258-2292), mute
@@ -659,7 +690,17 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
         }
     }
 
-    if (s->mute) {
+    const bool fwi = o.observed != nullptr;
+    if (fwi) { /* adsource (P-S-seperating.cu:1484-1498): residual on every receiver and step */
+        double m2 = 0;
+        for (int q = 0; q < 2; ++q)
+            for (size_t k = 0; k < (size_t)nx * nt; ++k) {
+                rec[q][k] = rec[q][k] - (R)o.observed[(size_t)q * nx * nt + k];
+                m2 += (double)rec[q][k] * (double)rec[q][k];
+            }
+        if (o.misfit) *o.misfit = 0.5 * m2;
+    }
+    if (s->mute && !fwi) {
         int t0 = ricker_t0(s->fm, p->dt);
         float vsrc = (float)S.vp_d[(size_t)Sx * Zn + Zr];
         for (int ix = 0; ix < nx; ++ix) {
@@ -690,6 +731,13 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
     std::vector<R> snew[4], now[4], past[4];
     for (int q = 0; q < 4; ++q) { snew[q] = F.cur[q]; now[q] = F.cur[q]; past[q] = F.prev[q]; }
     std::vector<R> num_pp(n, (R)0), num_ps(n, (R)0), den(n, (R)0);
+    /* FWI: source totals at level t+1 for backward step t (V^nt first), and
+     * the gradient accumulators */
+    std::vector<R> u1[2], gvp, gvs, ill;
+    if (fwi) {
+        u1[0] = F.vx; u1[1] = F.vz;
+        gvp.assign(n, (R)0); gvs.assign(n, (R)0); ill.assign(n, (R)0);
+    }
     double vmax_snap = 0;
     if (o.recon_err) {
         for (int t = 1; t <= nt; ++t)
@@ -732,6 +780,19 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
             Rw.vz[k] = rec[1][(size_t)ix * nt + t];
         }
         stress_cpml(S, Rw, -1, -1, (R)0);
+        if (fwi) {
+            gradfun_step(S, u1[0].data(), u1[1].data(), Rw.th.data(), Rw.om.data(), Rw.uxx.data(), Rw.uxz.data(),
+                         Rw.uzx.data(), Rw.uzz.data(), gvp.data(), gvs.data(), ill.data());
+            if (t == nt - 1) { /* level nt-1: the forward's own totals */
+                for (size_t k = 0; k < n; ++k) {
+                    u1[0][k] = F.prev[0][k] + F.prev[2][k];
+                    u1[1][k] = F.prev[1][k] + F.prev[3][k];
+                }
+            } else { /* level t: the reconstructed totals with the ring restored */
+                u1[0] = Ssrc.vx;
+                u1[1] = Ssrc.vz;
+            }
+        }
         velocity_cpml(S, Rw);
         /* corr_v (P-S-seperating.cu:1426-1439) */
         const std::vector<R> *sp = exact ? now : snew;
@@ -745,6 +806,14 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
                 num_ps[k] += spx * Rw.cur[2][k] + spz * Rw.cur[3][k];
             }
     }
+    if (fwi)
+        for (int i = L; i < g.Xn - L; ++i)
+            for (int j = L; j < Zn - L; ++j) {
+                size_t k = (size_t)i * Zn + j, oi = (size_t)(i - L) * nz + (j - L);
+                if (o.grad_vp) o.grad_vp[oi] = (double)gvp[k];
+                if (o.grad_vs) o.grad_vs[oi] = (double)gvs[k];
+                if (o.illum) o.illum[oi] = (double)ill[k];
+            }
     /* image_fun (P-S-seperating.cu:1441-1455) and Laplace (P-S-seperating.cu:1457-1469) */
     std::vector<R> ipp(n, (R)0), ips(n, (R)0);
     for (int i = L; i < g.Xn - L; ++i)
@@ -893,6 +962,28 @@ int oracle_rtm_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, 
     Outputs o;
     o.img_pp = img_pp; o.img_ps = img_ps; o.lap_pp = lap_pp; o.lap_ps = lap_ps; o.rec_out = rec_out;
     return precision == 32 ? run_shot<float>(p, m, s, rec_in, true, o) : run_shot<double>(p, m, s, rec_in, true, o);
+}
+
+/* FWI gradient of one shot: observed records [2][nx][nt]; grad_vp, grad_vs,
+ * illum are nx*nz (x-major); misfit = 0.5 * sum of squared residuals. */
+int oracle_fwi_gradient(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision,
+                        const double *observed, double *grad_vp, double *grad_vs, double *illum, double *misfit) {
+    if (!p || !m || !s || !observed) return OUC_INVALID_ARG;
+    Outputs o;
+    o.observed = observed;
+    o.grad_vp = grad_vp; o.grad_vs = grad_vs; o.illum = illum; o.misfit = misfit;
+    return precision == 32 ? run_shot<float>(p, m, s, nullptr, true, o) : run_shot<double>(p, m, s, nullptr, true, o);
+}
+
+/* One gradfun step on given padded Xn x Zn fields (FP64); the accumulators
+ * are updated in place over the interior. */
+int oracle_gradfun(const ouc_params *p, const ouc_model *m, const double *ux, const double *uz, const double *th,
+                   const double *om, const double *uxx, const double *uxz, const double *uzx, const double *uzz,
+                   double *gvp, double *gvs, double *ill) {
+    Setup<double> S;
+    if (!make_setup(p, m, nullptr, S)) return OUC_INVALID_ARG;
+    gradfun_step(S, ux, uz, th, om, uxx, uxz, uzx, uzz, gvp, gvs, ill);
+    return OUC_OK;
 }
 
 /* Reversibility: per backward step, max |reconstructed Vpx^t - forward Vpx^t|

@@ -42,6 +42,8 @@ def lib():
         L.oracle_forward_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
         L.oracle_rtm_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp, dp]
         L.oracle_recon_error.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
+        L.oracle_fwi_gradient.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp]
+        L.oracle_gradfun.argtypes = [P(abi.Params), P(abi.Model)] + [dp] * 11
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -166,6 +168,31 @@ def rtm_shot(p, vp, vs, shot, precision=64, rec_in=None):
     _check(lib().oracle_rtm_shot(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(rin),
                                  *[abi.dptr(a) for a in outs], abi.dptr(rec_out)), "rtm")
     return {"img_pp": outs[0], "img_ps": outs[1], "lap_pp": outs[2], "lap_ps": outs[3], "rec": rec_out}
+
+
+def fwi_gradient(p, vp, vs, shot, observed, precision=64):
+    """gradfun accumulated over one shot's backward loop, driven by the
+    residual modelled - observed (adsource); observed is [2][nx][nt]."""
+    m = _ModelRef(vp, vs)
+    s = _ShotRef(**shot)
+    obs = np.ascontiguousarray(observed, np.float64)
+    assert obs.shape == (2, p.nx, p.nt)
+    outs = [np.zeros((p.nx, p.nz), np.float64) for _ in range(3)]
+    mis = np.zeros(1, np.float64)
+    _check(lib().oracle_fwi_gradient(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(obs),
+                                     *[abi.dptr(a) for a in outs], abi.dptr(mis)), "fwi")
+    return {"grad_vp": outs[0], "grad_vs": outs[1], "illum": outs[2], "misfit": float(mis[0])}
+
+
+def gradfun(p, vp, vs, fields, acc):
+    """One gradfun step on padded (Xn, Zn) FP64 fields ux, uz, th, om, uxx,
+    uxz, uzx, uzz; acc = [gvp, gvs, ill] is updated in place."""
+    m = _ModelRef(vp, vs)
+    f = [np.ascontiguousarray(fields[k], np.float64) for k in ("ux", "uz", "th", "om", "uxx", "uxz", "uzx", "uzz")]
+    for a in acc:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    _check(lib().oracle_gradfun(C.byref(p), C.byref(m.m), *[abi.dptr(a) for a in f], *[abi.dptr(a) for a in acc]),
+           "gradfun")
 
 
 def recon_error(p, vp, vs, shot, precision=64):

@@ -614,6 +614,25 @@ ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, int32
     }
 }
 
+ouc_status ouc_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, const float *observed,
+                            float *grad_vp, float *grad_vs, float *illum, double *misfit) {
+    if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
+    if (nshots < 1 || !shots || !observed) return fail(c, OUC_INVALID_ARG, "need shots and their observed records");
+    if (c->p.engine != OUC_ENGINE_FUSED) return fail(c, OUC_UNSUPPORTED, "the FWI gradient runs on the fused engine");
+    return guarded(c, [&] { fused_fwi_gradient(c, shots, nshots, observed, grad_vp, grad_vs, illum, misfit); });
+}
+
+ouc_status ouc_model_update(int32_t device, int64_t n, const float *v, const float *grad, double step, int32_t iter,
+                            float *out) {
+    if (n < 1 || !v || !grad || !out || iter < 0) return fail(nullptr, OUC_INVALID_ARG, "bad arguments");
+    try {
+        fused_model_update(device, n, v, grad, step, iter, out);
+        return OUC_OK;
+    } catch (const Error &e) {
+        return fail(nullptr, e.code, e.what());
+    }
+}
+
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t) {
     if (!c || !t) return OUC_INVALID_ARG;
     *t = c->last;

@@ -16,6 +16,7 @@
 #include "fused.hpp"
 #include "edge_kernels.hpp"
 #include "fused_kernels.hpp"
+#include "fwi_kernels.hpp"
 #include "lapfft.hpp"
 #include "lpad_kernels.hpp"
 
@@ -92,6 +93,9 @@ struct FusedState {
     int csx = kCoreSX, pf = kCorePF; // core tile
     float lb_beta = 0.01f;           // learned pad damping (the default keeps 8000-step runs stable)
     DevBuf rec6;           // [6][nt][nx] forward seismograms of one shot
+    // FWI: gradient accumulators [3][cap][plane] (vp, vs, illumination), observed
+    // records [cap][2][nt][nx], residual partial sums, shot-summed interior outputs
+    DevBuf gacc, obs, part, gout;
     // time loops captured as HIP graphs, replayed while the key matches
     struct LoopGraph {
         int B = -1, rtm = -1, ncomp = -1;
@@ -514,8 +518,16 @@ struct Snap {
 // Runs B shots through forward (+ strips) and, with rtm, the backward loop;
 // leaves the accumulators of every shot in st->acc.  With the learned
 // boundary, st->mon holds each shot's boundary monitor afterwards.
+// FWI inputs of a batch: observed records [B][2][nx][nt] on the host; the
+// backward loop is driven by the residual and accumulates the gradient.
+struct FwiBatch {
+    const float *observed;
+    int rblocks;          // residual blocks per shot (partial sums in st->part)
+};
+
 void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe,
-               const Snap *snap = nullptr, const float *rec_in = nullptr, const Snap *rsnap = nullptr) {
+               const Snap *snap = nullptr, const float *rec_in = nullptr, const Snap *rsnap = nullptr,
+               const FwiBatch *fwi = nullptr) {
     FusedState *st = fs(c);
     const Grid &g = c->g;
     const FGrid &f = st->g;
@@ -540,7 +552,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         std::copy(w.begin(), w.end(), wavs.begin() + (size_t)b * nt + start[b]);
         if (zr < 0) zr = sh.rec_z + g.L;
         if (sh.rec_z + g.L != zr) throw Error(OUC_INVALID_ARG, "all shots of a batch must share the receiver row");
-        if (sh.mute) {
+        if (sh.mute && !fwi) {
             std::vector<int32_t> cnt;
             mute_counts(g, sh, c->vp_pad[(size_t)(sh.sx + g.L) * g.P + zr], cnt);
             for (int ix = 0; ix < g.nx; ++ix) counts[(size_t)b * g.nx + ix] = std::min(nt, cnt[ix] + start[b]);
@@ -608,6 +620,20 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         OUC_HIP(hipMemcpyAsync(st->rec.ptr, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice, s));
         OUC_HIP(hipStreamSynchronize(s));
     }
+    if (fwi) { // adsource (P-S-seperating.cu:1484-1498): modelled - observed on every receiver and step
+        std::vector<float> tr((size_t)B * 2 * nt * g.nx);
+        for (int b = 0; b < B; ++b)
+            for (int q = 0; q < 2; ++q)
+                for (int ix = 0; ix < g.nx; ++ix) {
+                    const float *src = fwi->observed + (((size_t)b * 2 + q) * g.nx + ix) * nt;
+                    float *dst = tr.data() + ((size_t)b * 2 + q) * nt * g.nx + ix;
+                    for (int t = 0; t < nt; ++t) dst[(size_t)t * g.nx] = src[t];
+                }
+        OUC_HIP(hipMemcpyAsync(st->obs.ptr, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        k_fresidual<<<dim3(fwi->rblocks, B), 256, 0, s>>>((long)2 * nt * g.nx, st->rec.f(), st->obs.f(),
+                                                           st->part.as<double>());
+        OUC_HIP(hipStreamSynchronize(s)); // the host buffer goes out of scope
+    }
     c->last.forward_ms = 0; // filled by the caller from events
     OUC_HIP(hipEventRecord(c->ev[1], s));
     {
@@ -620,6 +646,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     int rbs = 1 - rb;
     OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
     for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, span, s));
+    const size_t gspan = (size_t)st->plane * st->cap; // floats per gradient accumulator
+    if (fwi) OUC_HIP(hipMemsetAsync(st->gacc.ptr, 0, 3 * gspan * sizeof(float), s));
     int krs = 0;
     FStep r{};
     r.zr = zr;
@@ -637,7 +665,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     // stream: iteration t decodes slot t, and the reconstruction of step t reads
     // slots t+1 and (EXACT) t
     const dim3 dgrid(rblocks, RS_NF, B);
-    run_loop(s, graph_b, st->gbwd, B, rtm, ncomp, [&] {
+    run_loop(s, graph_b, st->gbwd, B, fwi ? 2 : 1, ncomp, [&] {
     for (int t = nt - 1; t >= 0; --t) {
         if (graph_b) {
             OUC_HIP(hipEventRecord(st->evf, s));
@@ -688,6 +716,23 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         if (graph_b) {
             OUC_HIP(hipEventRecord(st->evj, sr));
             OUC_HIP(hipStreamWaitEvent(s, st->evj, 0));
+        }
+        if (fwi) {
+            // gradfun on this step: source V^{t+1} (the buffer this step's reconstruction
+            // read, ring totals from the same slot), receiver V before its update with the
+            // injected residual; the next iteration's reconstruction overwrites the source
+            // buffer only after this launch (it forks from this stream)
+            FRecon grc{};
+            grc.ring = kr;
+            grc.strips = ks0;
+            grc.stride = sstride;
+            grc.slot_v = t >= nt - 2 ? -1 : (exact ? t : t + 1);
+            FStep ga = r;
+            ga.rb = 1 - rbr;
+            const dim3 gg((g.nz + kGradTZ - 1) / kGradTZ, (g.nx + kGradTX - 1) / kGradTX, B);
+            float *ga0 = st->gacc.f();
+            OUC_FDISPATCH(f.N, k_fgrad<NN><<<gg, dim3(64, 4), 0, s>>>(f, st->S, 1 - rbs, grc, st->R, ga, c->p2.f(),
+                                                                     c->q2.f(), ga0, ga0 + gspan, ga0 + 2 * gspan));
         }
     }
     OUC_HIP(hipGetLastError());
@@ -915,6 +960,85 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *im
     c->last.cell_updates = (double)g.nt * (2 * padded + interior);
     c->last.canonical_bytes = (double)g.nt * (2 * (120 * interior + 232 * cpml) + 120 * interior + 48 * interior);
     c->last.own_bytes = fused_own_bytes(c);
+}
+
+// FWI gradient summed over shots (gradfun with the adsource residual); outputs
+// are x-major interior arrays, misfit = 0.5 * sum of squared residuals.
+void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const float *observed, float *grad_vp,
+                        float *grad_vs, float *illum, double *misfit) {
+    const Grid &g = c->g;
+    if (g.learned) throw Error(OUC_UNSUPPORTED, "the FWI gradient runs with the CPML boundary");
+    for (int k = 0; k < nshots; ++k) {
+        const ouc_shot *sh = &shots[k];
+        if (sh->sx < 0 || sh->sx >= g.nx || sh->sz < 0 || sh->sz >= g.nz || sh->rec_z < 0 || sh->rec_z >= g.nz)
+            throw Error(OUC_INVALID_ARG, "shot " + std::to_string(k) + " outside the interior");
+        if (!sh->wavelet && !(sh->fm > 0)) throw Error(OUC_INVALID_ARG, "fm must be positive");
+        if (sh->nt > 0 && sh->nt != g.nt) throw Error(OUC_UNSUPPORTED, "the FWI gradient needs every shot at the context's nt");
+    }
+    // capacity: the per-shot state plus the gradient accumulators and observed records
+    const size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
+    const size_t extra = (size_t)3 * g.pitched() * sizeof(float) + (size_t)2 * g.nt * g.nx * sizeof(float);
+    int cap = default_capacity(c, nshots);
+    cap = std::max(1, (int)((double)cap * per / (per + extra)));
+    ensure(c, cap);
+    FusedState *st = fs(c);
+    cap = std::min(cap, st->cap);
+    const size_t gbytes = (size_t)3 * st->plane * st->cap * sizeof(float);
+    const size_t obytes = (size_t)st->cap * 2 * g.nt * g.nx * sizeof(float);
+    const int rblocks = 64;
+    if (st->gacc.bytes < gbytes || st->obs.bytes < obytes) {
+        st->drop_graphs(); // captured loops hold the old addresses
+        if (st->gacc.bytes < gbytes) st->gacc.alloc(gbytes);
+        if (st->obs.bytes < obytes) st->obs.alloc(obytes);
+    }
+    if (st->part.bytes < (size_t)st->cap * rblocks * sizeof(double)) st->part.alloc((size_t)st->cap * rblocks * sizeof(double));
+    const size_t ni = (size_t)g.nx * g.nz;
+    if (st->gout.bytes < 3 * ni * sizeof(float)) st->gout.alloc(3 * ni * sizeof(float));
+    OUC_HIP(hipMemsetAsync(st->gout.ptr, 0, 3 * ni * sizeof(float), c->stream));
+    c->last = ouc_timings{};
+    CoreProbe probe;
+    OUC_HIP(hipEventRecord(c->ev[0], c->stream));
+    double m2 = 0;
+    std::vector<double> part;
+    const size_t gspan = (size_t)st->plane * st->cap;
+    for (int b0 = 0; b0 < nshots; b0 += cap) {
+        const int B = std::min(cap, nshots - b0);
+        FwiBatch fb{observed + (size_t)b0 * 2 * g.nx * g.nt, rblocks};
+        run_batch(c, shots + b0, B, true, 2, probe, nullptr, nullptr, nullptr, &fb);
+        const dim3 grid((g.nz + 255) / 256, g.nx);
+        for (int q = 0; q < 3; ++q)
+            k_shot_sum<<<grid, 256, 0, c->stream>>>(st->g, B, st->gacc.f() + q * gspan, st->gout.f() + q * ni);
+        OUC_HIP(hipGetLastError());
+        part.resize((size_t)B * rblocks);
+        OUC_HIP(hipMemcpyAsync(part.data(), st->part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        OUC_HIP(hipStreamSynchronize(c->stream));
+        for (double v : part) m2 += v;
+    }
+    float *outs[3] = {grad_vp, grad_vs, illum};
+    for (int q = 0; q < 3; ++q)
+        if (outs[q])
+            OUC_HIP(hipMemcpyAsync(outs[q], st->gout.f() + q * ni, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    OUC_HIP(hipEventRecord(c->ev[5], c->stream));
+    OUC_HIP(hipStreamSynchronize(c->stream));
+    float tot = 0;
+    OUC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+    c->last.total_ms = tot;
+    if (misfit) *misfit = 0.5 * m2;
+}
+
+void fused_model_update(int device, int64_t n, const float *v, const float *grad, double step, int iter, float *out) {
+    OUC_HIP(hipSetDevice(device));
+    DevBuf dv, dg, dout;
+    dv.alloc((size_t)n * sizeof(float));
+    dg.alloc((size_t)n * sizeof(float));
+    dout.alloc((size_t)n * sizeof(float));
+    OUC_HIP(hipMemcpy(dv.ptr, v, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    OUC_HIP(hipMemcpy(dg.ptr, grad, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    const double scale = step * std::pow(0.95, iter); // a_2 * pow(0.95, k), P-S-seperating.cu:1511
+    k_model_update<<<(unsigned)((n + 255) / 256), 256>>>((long)n, dv.f(), dg.f(), scale, dout.f());
+    OUC_HIP(hipGetLastError());
+    OUC_HIP(hipMemcpy(out, dout.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
 }
 
 // Forward seismograms [6][nx][nt] of one shot through the fused path.

@@ -15,6 +15,11 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *im
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms);
 double fused_own_bytes(const ouc_ctx *c);
+// FWI gradient (gradfun with the adsource residual) summed over shots.
+void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const float *observed, float *grad_vp,
+                        float *grad_vs, float *illum, double *misfit);
+// revise_model_end on one array (device memory held for the call only).
+void fused_model_update(int device, int64_t n, const float *v, const float *grad, double step, int iter, float *out);
 // Forward seismograms [6][nx][nt] of one shot through the fused kernels.
 void fused_forward_one(ouc_ctx *c, const ouc_shot *s, float *seis);
 // Forward run with V snapshots over a padded-grid rectangle after each listed step.

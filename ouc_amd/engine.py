@@ -49,6 +49,10 @@ def lib():
     L.ouc_write_image_files.restype = C.c_int
     L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8), fp]
     L.ouc_strip_codec_fp16.restype = C.c_int
+    L.ouc_fwi_gradient.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, fp, fp, fp, fp, P(C.c_double)]
+    L.ouc_fwi_gradient.restype = C.c_int
+    L.ouc_model_update.argtypes = [C.c_int32, C.c_int64, fp, fp, C.c_double, C.c_int32, fp]
+    L.ouc_model_update.restype = C.c_int
     L.ouc_set_batch_capacity.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
     L.ouc_set_batch_capacity.restype = C.c_int
     L.ouc_set_tile.argtypes = [C.c_void_p, P(abi.Tile)]
@@ -179,6 +183,24 @@ class Engine:
             out["per_shot_ms"] = ms
         return out
 
+    def fwi_gradient(self, shots, observed):
+        """Elastic FWI gradient summed over `shots` (fused engine): observed is
+        [nshots][2][nx][nt] (a single shot may pass [2][nx][nt]).  Returns
+        grad_vp, grad_vs, illum (nx, nz) and the misfit 0.5 * sum(residual^2)."""
+        n = len(shots)
+        arr = (abi.Shot * n)()
+        keep = []
+        for k, sh in enumerate(shots):
+            s, w = _shot_struct(sh, self.p.nt)
+            arr[k] = s
+            keep.append(w)
+        obs = np.ascontiguousarray(observed, np.float32).reshape(n, 2, self.p.nx, self.p.nt)
+        outs = [np.zeros((self.p.nx, self.p.nz), np.float32) for _ in range(3)]
+        mis = C.c_double()
+        self._check(lib().ouc_fwi_gradient(self._h, arr, n, abi.fptr(obs), *[abi.fptr(a) for a in outs],
+                                           C.byref(mis)))
+        return {"grad_vp": outs[0], "grad_vs": outs[1], "illum": outs[2], "misfit": mis.value}
+
     def set_batch_capacity(self, shots):
         """At most `shots` shots per fused launch (0: by free memory); returns
         the device bytes one shot of a batch holds."""
@@ -282,6 +304,20 @@ def strip_codec_fp16(x, device=0, decoded=False):
         raise OucError(st, lib().ouc_last_error(None).decode())
     out = (h, e, d) if decoded else (h, e)
     return tuple(a[0] for a in out) if x.ndim == 1 else out
+
+
+def model_update(v, grad, step=30.0, iteration=0, device=0):
+    """revise_model_end on the GPU: v - step * 0.95^iteration * grad, in double,
+    rounded to float."""
+    v = np.ascontiguousarray(v, np.float32)
+    g = np.ascontiguousarray(grad, np.float32)
+    assert v.shape == g.shape
+    out = np.zeros_like(v)
+    st = lib().ouc_model_update(int(device), v.size, abi.fptr(v), abi.fptr(g), float(step), int(iteration),
+                                abi.fptr(out))
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
+    return out
 
 
 def _lb_check(st):
