@@ -49,6 +49,8 @@ def parse():
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
     ap.add_argument("--serve-shots", type=int, default=32)
     ap.add_argument("--serve-schedule", choices=["sjf", "concurrent"], default="sjf")
+    ap.add_argument("--fwi", action="store_true",
+                    help="FWI gradient throughput (shots/hour) on the same workload instead of RTM")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -84,6 +86,9 @@ def main():
         return
     if a.serve:
         serve_bench(a)
+        return
+    if a.fwi:
+        fwi_bench(a)
         return
     from ouc_amd import abi, velocity_models as vm
     from ouc_amd.engine import Engine, OucError
@@ -193,6 +198,45 @@ def main():
         "batched_over_plain": (shots_per_hour / plain["shots_per_hour"]) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def fwi_bench(a):
+    """FWI gradient (residual-driven backward loop with the gradient kernel)
+    over batches of shots; observed records are zero, so the residual is the
+    modelled record (the work does not depend on the values)."""
+    import numpy as np
+
+    from ouc_amd import abi, tuning, velocity_models as vm
+    from ouc_amd.engine import Engine
+
+    strips = abi.STRIP_FP16 if a.strips == "fp16" else abi.STRIP_FP32
+    p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=abi.ENGINE_FUSED, strip_precision=strips)
+    batch = a.batch or 32
+    for sh in shots:
+        sh["mute"] = 0
+    log = lambda m: print(f"[fwi] {m}", file=sys.stderr, flush=True)
+    log(f"batch {batch}, {a.steps} timed + {a.warmup} warmup steps, grid {p.nx}x{p.nz} nt={p.nt}")
+    obs = np.zeros((batch, 2, p.nx, p.nt), np.float32)
+    with Engine(p, vp, vs) as e:
+        cols, pf, _ = tuning.tuned(e, batch)
+        for w in range(a.warmup):
+            e.fwi_gradient(shots[:batch], obs)
+            log(f"warmup step {w} done")
+        dev_ms = 0.0
+        for s_ in range(a.steps):
+            e.fwi_gradient(shots[:batch], obs)
+            dev_ms += e.timings()["total_ms"]
+            log(f"step {s_}: {e.timings()['total_ms']:.1f} ms device")
+    nshots = a.steps * batch
+    line = {
+        "metric": "FWI gradient shots/hour at 2048x1024 (one GPU)", "value": nshots / (dev_ms / 1e3) * 3600.0,
+        "unit": "shots/hour", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dev_ms / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered model, zero observed records)",
+        "config": {"workload": f"FWI gradient, 2048x1024 interior, nt {p.nt}, {batch} shots per step, "
+                               f"{a.strips} strips, tile {cols}x{pf}"},
     }
     print(json.dumps(line))
 

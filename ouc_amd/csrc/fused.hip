@@ -729,7 +729,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             grc.slot_v = t >= nt - 2 ? -1 : (exact ? t : t + 1);
             FStep ga = r;
             ga.rb = 1 - rbr;
-            const dim3 gg((g.nz + kGradTZ - 1) / kGradTZ, (g.nx + kGradTX - 1) / kGradTX, B);
+            const dim3 gg(grad_blocks(g.nx, g.nz), 1, B);
             float *ga0 = st->gacc.f();
             OUC_FDISPATCH(f.N, k_fgrad<NN><<<gg, dim3(64, 4), 0, s>>>(f, st->S, 1 - rbs, grc, st->R, ga, c->p2.f(),
                                                                      c->q2.f(), ga0, ga0 + gspan, ga0 + 2 * gspan));
@@ -757,15 +757,15 @@ void shot_images(ouc_ctx *c, FusedState *st, int b) {
     OUC_HIP(hipGetLastError());
 }
 
-int default_capacity(ouc_ctx *c, int nshots) {
+int default_capacity(ouc_ctx *c, int nshots, size_t extra_per_shot = 0) {
     size_t freeb = 0, total = 0;
     OUC_HIP(hipMemGetInfo(&freeb, &total));
     // keep what this context already holds available for reuse
     FusedState *st = fs(c);
     size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes +
-                  st->keyb.bytes + st->ef.bytes + st->decb.bytes;
+                  st->keyb.bytes + st->ef.bytes + st->decb.bytes + st->gacc.bytes + st->obs.bytes;
     size_t avail = (size_t)((freeb + held) * 0.85);
-    size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
+    size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16) + extra_per_shot;
     int cap = (int)std::max<size_t>(1, avail / per);
     if (c->batch_cap > 0) cap = std::min(cap, c->batch_cap);
     return std::min(cap, nshots);
@@ -976,10 +976,8 @@ void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const flo
         if (sh->nt > 0 && sh->nt != g.nt) throw Error(OUC_UNSUPPORTED, "the FWI gradient needs every shot at the context's nt");
     }
     // capacity: the per-shot state plus the gradient accumulators and observed records
-    const size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16);
     const size_t extra = (size_t)3 * g.pitched() * sizeof(float) + (size_t)2 * g.nt * g.nx * sizeof(float);
-    int cap = default_capacity(c, nshots);
-    cap = std::max(1, (int)((double)cap * per / (per + extra)));
+    int cap = default_capacity(c, nshots, extra);
     ensure(c, cap);
     FusedState *st = fs(c);
     cap = std::min(cap, st->cap);

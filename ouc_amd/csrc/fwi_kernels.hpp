@@ -32,6 +32,13 @@ __global__ __launch_bounds__(256) void k_fresidual(long per_shot, float *__restr
 // Tile of the gradient kernel: 64 rows along z by 16 columns along x.
 constexpr int kGradTZ = 64, kGradTX = 16;
 
+// Blocks along x of the gradient grid for an interior of nx x nz: a multiple
+// of 8, so that block b of a shot runs on XCD b mod 8 whatever the shot.
+__host__ __device__ inline int grad_tiles(int nx, int nz) {
+    return ((nz + kGradTZ - 1) / kGradTZ) * ((nx + kGradTX - 1) / kGradTX);
+}
+__host__ __device__ inline int grad_blocks(int nx, int nz) { return (grad_tiles(nx, nz) + 7) / 8 * 8; }
+
 // One backward step of gradfun on the interior [L, Xn-L) x [L, Zn-L).
 // Source: the reconstructed totals V^{t+1} as the reconstruction's stress pass
 // of this step reads them (ring cells from the strips at rc.slot_v); receiver:
@@ -41,6 +48,9 @@ constexpr int kGradTZ = 64, kGradTX = 16;
 // stencil (the reference's N = 6 duxdz at P-S-seperating.cu:1556 pairs f[j+4]
 // with f[j-6]; this is the plain Dp_z).  The live physics has no density, so
 // the reference's factor 2 P vp is 2 vp.  Accumulators are [shot][plane].
+// Blocks are dispatched round-robin over the 8 XCDs, so block b takes tile
+// (b mod 8) * T + b / 8 (z fastest): each XCD streams a contiguous run of
+// tiles and the halos its neighbours read are in its own L2.
 template <int N>
 __global__ __launch_bounds__(256) void k_fgrad(FGrid g, FWave src, int sbuf, FRecon rc, FWave rcv, FStep a,
                                                const float *__restrict__ p2, const float *__restrict__ q2,
@@ -48,7 +58,10 @@ __global__ __launch_bounds__(256) void k_fgrad(FGrid g, FWave src, int sbuf, FRe
                                                float *__restrict__ ill) {
     constexpr int HX = kGradTX + 2 * N, HZ = kGradTZ + 2 * N;
     __shared__ float sx_[HX][HZ], sz_[HX][HZ], rx_[HX][HZ], rz_[HX][HZ];
-    const int i0 = g.L + blockIdx.y * kGradTX, j0 = g.L + blockIdx.x * kGradTZ;
+    const int ntz = (g.nz + kGradTZ - 1) / kGradTZ, per_xcd = gridDim.x / 8;
+    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (tile >= grad_tiles(g.nx, g.nz)) return;
+    const int i0 = g.L + (tile / ntz) * kGradTX, j0 = g.L + (tile % ntz) * kGradTZ;
     const long so = (long)blockIdx.z * g.plane;
     const KStrips rs = shot_strips(rc);
     const int tid = threadIdx.y * 64 + threadIdx.x;
