@@ -245,6 +245,36 @@ ouc_status ouc_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, c
 ouc_status ouc_model_update(int32_t device, int64_t n, const float *v, const float *grad, double step, int32_t iter,
                             float *out);
 
+/* Model dump: dir/Vp.dat and dir/Vs.dat, the interior as raw little-endian
+ * float32, x-major (the layout of the reference's model dump, kept commented
+ * out at P-S-seperating.cu:2042-2063).  Creates dir; either array may be NULL. */
+ouc_status ouc_write_model_files(const char *dir, int32_t nx, int32_t nz, const float *vp, const float *vs);
+
+/* One shot gather of one component as a SEG-Y revision 1 file.  The
+ * reference records seismograms (P-S-seperating.cu:307-311) but never writes
+ * them.  Layout: a 3200-byte EBCDIC textual header, a 400-byte binary header
+ * (sample format 5 = IEEE float, fixed-length traces), then per receiver a
+ * 240-byte trace header (sequence numbers, field record = shot, offset,
+ * source/receiver X, depths as elevations, coordinate scalar -100) and nt
+ * big-endian float samples.  traces: [ntraces][nt] floats, receiver-major
+ * like a component of the seismogram arrays. */
+typedef struct ouc_segy_gather {
+    int32_t shot;       /* field record number */
+    int32_t ntraces;    /* receivers, one trace each */
+    int32_t nt;         /* samples per trace, at most 32767 */
+    float dt;           /* sample interval, seconds (1 us .. 32.767 ms) */
+    float sx_m, sz_m;   /* source position, metres */
+    float rec_x0_m, rec_dx_m, rec_z_m; /* receiver k at x = rec_x0 + k rec_dx, depth rec_z */
+    char component[8];  /* written to the textual header, e.g. "VX" */
+} ouc_segy_gather;
+ouc_status ouc_write_segy(const char *path, const ouc_segy_gather *g, const float *traces);
+
+/* Reads a file written by ouc_write_segy (or any fixed-length, format-5,
+ * revision-1 file without extended textual headers).  With traces NULL,
+ * returns the sizes only; otherwise fills [ntraces][nt] floats, capacity
+ * being the buffer's length in floats.  dt may be NULL. */
+ouc_status ouc_read_segy(const char *path, int32_t *ntraces, int32_t *nt, float *dt, float *traces, int64_t capacity);
+
 /* Timing of the last forward/RTM/batch call on this context. */
 ouc_status ouc_get_timings(const ouc_ctx *c, ouc_timings *t);
 

@@ -64,6 +64,14 @@ class Timings(C.Structure):
     ]
 
 
+class SegyGather(C.Structure):
+    _fields_ = [
+        ("shot", C.c_int32), ("ntraces", C.c_int32), ("nt", C.c_int32), ("dt", C.c_float),
+        ("sx_m", C.c_float), ("sz_m", C.c_float), ("rec_x0_m", C.c_float), ("rec_dx_m", C.c_float),
+        ("rec_z_m", C.c_float), ("component", C.c_char * 8),
+    ]
+
+
 def make_params(nx, nz, npml, half_order, nt, dx, dz, dt, R=1e-3, semantics=SEM_FAITHFUL,
                 strip_precision=STRIP_FP32, boundary=BOUNDARY_CPML, laplacian=LAP_STENCIL,
                 engine=ENGINE_PLAIN, band=0):

@@ -49,6 +49,12 @@ def lib():
     L.ouc_write_image_files.restype = C.c_int
     L.ouc_strip_codec_fp16.argtypes = [C.c_int32, fp, C.c_int64, C.c_int32, P(C.c_uint16), P(C.c_int8), fp]
     L.ouc_strip_codec_fp16.restype = C.c_int
+    L.ouc_write_model_files.argtypes = [C.c_char_p, C.c_int32, C.c_int32, fp, fp]
+    L.ouc_write_model_files.restype = C.c_int
+    L.ouc_write_segy.argtypes = [C.c_char_p, P(abi.SegyGather), fp]
+    L.ouc_write_segy.restype = C.c_int
+    L.ouc_read_segy.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), fp, fp, C.c_int64]
+    L.ouc_read_segy.restype = C.c_int
     L.ouc_fwi_gradient.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, fp, fp, fp, fp, P(C.c_double)]
     L.ouc_fwi_gradient.restype = C.c_int
     L.ouc_model_update.argtypes = [C.c_int32, C.c_int64, fp, fp, C.c_double, C.c_int32, fp]
@@ -304,6 +310,37 @@ def strip_codec_fp16(x, device=0, decoded=False):
         raise OucError(st, lib().ouc_last_error(None).decode())
     out = (h, e, d) if decoded else (h, e)
     return tuple(a[0] for a in out) if x.ndim == 1 else out
+
+
+def _io_check(st):
+    if st != abi.OUC_OK:
+        raise OucError(st, lib().ouc_last_error(None).decode())
+
+
+def write_model_files(directory, vp=None, vs=None):
+    """directory/Vp.dat and Vs.dat: raw float32 interiors, x-major."""
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vp, vs)]
+    shape = next(a.shape for a in arrs if a is not None)
+    _io_check(lib().ouc_write_model_files(str(directory).encode(), shape[0], shape[1],
+                                          *[abi.fptr(a) for a in arrs]))
+
+
+def write_segy(path, traces, dt, shot=0, sx_m=0.0, sz_m=0.0, rec_x0_m=0.0, rec_dx_m=1.0, rec_z_m=0.0,
+               component="VX"):
+    """One component's shot gather [ntraces][nt] as a SEG-Y revision 1 file."""
+    tr = np.ascontiguousarray(traces, np.float32)
+    g = abi.SegyGather(int(shot), tr.shape[0], tr.shape[1], float(dt), float(sx_m), float(sz_m), float(rec_x0_m),
+                       float(rec_dx_m), float(rec_z_m), component.encode()[:7])
+    _io_check(lib().ouc_write_segy(str(path).encode(), C.byref(g), abi.fptr(tr)))
+
+
+def read_segy(path):
+    """(traces [ntraces][nt] float32, dt seconds) of a fixed-length IEEE-float SEG-Y file."""
+    n, ns, dt = C.c_int32(), C.c_int32(), C.c_float()
+    _io_check(lib().ouc_read_segy(str(path).encode(), C.byref(n), C.byref(ns), C.byref(dt), None, 0))
+    out = np.zeros((n.value, ns.value), np.float32)
+    _io_check(lib().ouc_read_segy(str(path).encode(), C.byref(n), C.byref(ns), C.byref(dt), abi.fptr(out), out.size))
+    return out, dt.value
 
 
 def model_update(v, grad, step=30.0, iteration=0, device=0):

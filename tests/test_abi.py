@@ -41,8 +41,9 @@ C_PROBE = r"""
 #include <stdio.h>
 #include "ouc.h"
 int main(void) {
-    printf("%zu %zu %zu %zu %zu %zu\n", sizeof(ouc_params), sizeof(ouc_model), sizeof(ouc_shot),
-           sizeof(ouc_timings), offsetof(ouc_params, R), offsetof(ouc_timings, boundary_ratio));
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(ouc_params), sizeof(ouc_model), sizeof(ouc_shot),
+           sizeof(ouc_timings), offsetof(ouc_params, R), offsetof(ouc_timings, boundary_ratio),
+           sizeof(ouc_segy_gather), offsetof(ouc_segy_gather, component));
     return 0;
 }
 """
@@ -58,7 +59,8 @@ def test_header_compiles_as_c99_and_layouts_match_ctypes(tmp_path):
                     str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert sizes == [C.sizeof(abi.Params), C.sizeof(abi.Model), C.sizeof(abi.Shot), C.sizeof(abi.Timings),
-                     abi.Params.R.offset, abi.Timings.boundary_ratio.offset]
+                     abi.Params.R.offset, abi.Timings.boundary_ratio.offset, C.sizeof(abi.SegyGather),
+                     abi.SegyGather.component.offset]
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libouc.so not built")
