@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--serve-schedule", choices=["sjf", "concurrent"], default="sjf")
     ap.add_argument("--fwi", action="store_true",
                     help="FWI gradient throughput (shots/hour) on the same workload instead of RTM")
+    ap.add_argument("--split-pml", action="store_true",
+                    help="forward modelling with the first-order split-PML engine (one shot per step)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -89,6 +91,9 @@ def main():
         return
     if a.fwi:
         fwi_bench(a)
+        return
+    if a.split_pml:
+        splitpml_bench(a)
         return
     from ouc_amd import abi, velocity_models as vm
     from ouc_amd.engine import Engine, OucError
@@ -237,6 +242,35 @@ def fwi_bench(a):
         "data": "synthetic (seeded layered model, zero observed records)",
         "config": {"workload": f"FWI gradient, 2048x1024 interior, nt {p.nt}, {batch} shots per step, "
                                f"{a.strips} strips, tile {cols}x{pf}"},
+    }
+    print(json.dumps(line))
+
+
+def splitpml_bench(a):
+    """Forward modelling of one shot per step with the first-order split-PML
+    engine (stress and velocity launches per time step) on the same grid."""
+    from ouc_amd import abi, velocity_models as vm
+    from ouc_amd.engine import Engine
+
+    p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=abi.ENGINE_SPLIT_PML)
+    log = lambda m: print(f"[split-pml] {m}", file=sys.stderr, flush=True)
+    with Engine(p, vp, vs) as e:
+        for w in range(a.warmup):
+            e.forward_shot(dict(shots[w % len(shots)], mute=0))
+            log(f"warmup step {w} done")
+        dev_ms = 0.0
+        for s_ in range(a.steps):
+            e.forward_shot(dict(shots[(a.warmup + s_) % len(shots)], mute=0))
+            dev_ms += e.timings()["total_ms"]
+            log(f"step {s_}: {e.timings()['total_ms']:.1f} ms device")
+    upd = (p.nx + 2 * p.npml - 2 * p.half_order) * (p.nz + 2 * p.npml - 2 * p.half_order)
+    line = {
+        "metric": "split-PML forward modelling, Gcell-updates/s at 2048x1024 (one GPU)",
+        "value": upd * p.nt * a.steps / (dev_ms / 1e3) / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": dev_ms / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded layered model)",
+        "config": {"workload": f"split-PML first-order forward, 2048x1024 interior, nt {p.nt}, one shot per step"},
+        "shots_per_hour": a.steps / (dev_ms / 1e3) * 3600.0,
     }
     print(json.dumps(line))
 

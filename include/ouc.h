@@ -73,8 +73,14 @@ enum { OUC_LAP_STENCIL = 0, OUC_LAP_FFT5 = 1, OUC_LAP_FFT_CONT = 2 };
 /* Kernel structure.  PLAIN keeps the reference's kernel split (separate stress
  * pass, velocity pass, strip copy, reconstruction pair and imaging kernels,
  * P-S-seperating.cu:2258-2364).  FUSED runs one single-pass kernel per
- * wavefield per step with strip I/O and the imaging condition folded in. */
-enum { OUC_ENGINE_PLAIN = 0, OUC_ENGINE_FUSED = 1 };
+ * wavefield per step with strip I/O and the imaging condition folded in.
+ * SPLIT_PML is the reference's other formulation, defined but never launched
+ * by its main(): first-order velocity-stress with split-field PML damping
+ * (stress P-S-seperating.cu:487-555, velocity 390-485; 12th-order
+ * coefficients to five digits at N = 6).  It serves ouc_forward_shot only
+ * (CPML profiles of the context, no mute); the RTM and FWI calls return
+ * OUC_UNSUPPORTED for it. */
+enum { OUC_ENGINE_PLAIN = 0, OUC_ENGINE_FUSED = 1, OUC_ENGINE_SPLIT_PML = 2 };
 
 /* Run parameters.  Replaces the #defines L, N, NT, Xn, Zn, R
  * (P-S-seperating.cu:15-23) and the locals dt, dx, dz

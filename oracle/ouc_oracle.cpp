@@ -624,6 +624,79 @@ void gradfun_step(const Setup<R> &S, const R *ux, const R *uz, const R *th, cons
         }
 }
 
+/* First-order velocity-stress engine with split PML: `stress`
+ * (P-S-seperating.cu:487-555) then `velocity` (P-S-seperating.cu:390-485) per
+ * step, never launched by the reference's main().  The damping factors
+ * (1 / (1 + dt a / 2)) and (1 - dt a / 2) are evaluated in R (the reference
+ * mixes double literals into float expressions).  At N = 6 the coefficients are
+ * these kernels' five-digit literals.  Records: the six components at row Zr
+ * of every interior column after each velocity pass (the reference's
+ * `if (j = Z_receive)` assigns rather than compares, so every thread writes
+ * the receiver row). */
+template <class R>
+int run_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, double *seis) {
+    Setup<R> S;
+    if (!make_setup(p, m, s, S)) return OUC_INVALID_ARG;
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn, nt = g.nt, nx = g.nx, L = g.L;
+    if (s->sx < 0 || s->sx >= nx || s->sz < 0 || s->sz >= g.nz || s->rec_z < 0 || s->rec_z >= g.nz)
+        return OUC_INVALID_ARG;
+    R c[8];
+    const float a6[6] = {+1.2213e+0f, -9.6931e-2f, 1.7448e-2f, -2.9673e-3f, +3.5901e-4f, -2.1848e-5f};
+    for (int k = 0; k < N; ++k) c[k] = N == 6 ? (R)a6[k] : S.c[k];
+    const int Sx = s->sx + L, Sz = s->sz + L, Zr = s->rec_z + L;
+    const size_t n = (size_t)g.Xn * Zn;
+    std::vector<R> vx(n, 0), vz(n, 0), vpx(n, 0), vpz(n, 0), vsx(n, 0), vsz(n, 0), th(n, 0), om(n, 0);
+    std::vector<R> thx(n, 0), thz(n, 0), omx(n, 0), omz(n, 0), uxx(n, 0), uxz(n, 0), uzx(n, 0), uzz(n, 0);
+    const R dt = S.dt, half = (R)0.5, one = (R)1;
+    for (int t = 0; t < nt; ++t) {
+#pragma omp parallel for schedule(static)
+        for (int i = N; i < g.Xn - N; ++i)
+            for (int j = N; j < Zn - N; ++j) {
+                size_t k = (size_t)i * Zn + j;
+                R dvxdx = dmx(vx.data(), k, Zn, c, N, S.dx), dvxdz = dpz(vx.data(), k, c, N, S.dz);
+                R dvzdz = dmz(vz.data(), k, c, N, S.dz), dvzdx = dpx(vz.data(), k, Zn, c, N, S.dx);
+                R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
+                R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
+                R p2 = S.vp[k] * S.vp[k], q2 = S.vs[k] * S.vs[k];
+                thx[k] = ax1 * (dt * p2 * dvxdx + ax2 * thx[k]);
+                thz[k] = az1 * (dt * p2 * dvzdz + az2 * thz[k]);
+                omx[k] = ax1 * (-dt * q2 * dvzdx + ax2 * omx[k]);
+                omz[k] = az1 * (dt * q2 * dvxdz + az2 * omz[k]);
+                uxx[k] = ax1 * (dt * q2 * dvxdx + ax2 * uxx[k]);
+                uxz[k] = az1 * (dt * q2 * dvxdz + az2 * uxz[k]);
+                uzx[k] = ax1 * (dt * q2 * dvzdx + ax2 * uzx[k]);
+                uzz[k] = az1 * (dt * q2 * dvzdz + az2 * uzz[k]);
+                th[k] = thx[k] + thz[k] + ((i == Sx && j == Sz) ? S.src[t] : (R)0);
+                om[k] = omx[k] + omz[k];
+            }
+#pragma omp parallel for schedule(static)
+        for (int i = N; i < g.Xn - N; ++i)
+            for (int j = N; j < Zn - N; ++j) {
+                size_t k = (size_t)i * Zn + j;
+                R dthdx = dpx(th.data(), k, Zn, c, N, S.dx), dthdz = dpz(th.data(), k, c, N, S.dz);
+                R domdx = dmx(om.data(), k, Zn, c, N, S.dx), domdz = dmz(om.data(), k, c, N, S.dz);
+                R duxxdz = dpz(uxx.data(), k, c, N, S.dz), duxzdx = dmx(uxz.data(), k, Zn, c, N, S.dx);
+                R duzxdz = dmz(uzx.data(), k, c, N, S.dz), duzzdx = dpx(uzz.data(), k, Zn, c, N, S.dx);
+                R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
+                R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
+                vpx[k] = ax1 * (ax2 * vpx[k] + dt * (dthdx + duzxdz - duzzdx));
+                vpz[k] = az1 * (az2 * vpz[k] + dt * (dthdz + duxzdx - duxxdz));
+                vsx[k] = ax1 * (ax2 * vsx[k] + dt * (domdz + duzxdz - duzzdx));
+                vsz[k] = az1 * (az2 * vsz[k] + dt * (-domdx + duxzdx - duxxdz));
+                vx[k] = vpx[k] + vsx[k];
+                vz[k] = vpz[k] + vsz[k];
+            }
+        if (seis)
+            for (int ix = 0; ix < nx; ++ix) {
+                size_t k = (size_t)(ix + L) * Zn + Zr;
+                const R *f[6] = {vx.data(), vz.data(), vpx.data(), vpz.data(), vsx.data(), vsz.data()};
+                for (int q = 0; q < 6; ++q) seis[((size_t)q * nx + ix) * nt + t] = (double)f[q][k];
+            }
+    }
+    return OUC_OK;
+}
+
 struct Outputs {
     double *seis = nullptr;      /* [6][nx][nt] */
     double *rec_out = nullptr;   /* [2][nx][nt] records used for RTM (after the mute) */
@@ -962,6 +1035,12 @@ int oracle_rtm_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, 
     Outputs o;
     o.img_pp = img_pp; o.img_ps = img_ps; o.lap_pp = lap_pp; o.lap_ps = lap_ps; o.rec_out = rec_out;
     return precision == 32 ? run_shot<float>(p, m, s, rec_in, true, o) : run_shot<double>(p, m, s, rec_in, true, o);
+}
+
+/* Split-PML first-order engine, forward seismograms [6][nx][nt]. */
+int oracle_forward_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision, double *seis) {
+    if (!p || !m || !s) return OUC_INVALID_ARG;
+    return precision == 32 ? run_splitpml<float>(p, m, s, seis) : run_splitpml<double>(p, m, s, seis);
 }
 
 /* FWI gradient of one shot: observed records [2][nx][nt]; grad_vp, grad_vs,

@@ -42,6 +42,7 @@ def lib():
         L.oracle_forward_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
         L.oracle_rtm_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp, dp]
         L.oracle_recon_error.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
+        L.oracle_forward_splitpml.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
         L.oracle_fwi_gradient.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp]
         L.oracle_gradfun.argtypes = [P(abi.Params), P(abi.Model)] + [dp] * 11
         L.oracle_set_threads.argtypes = [C.c_int]
@@ -168,6 +169,16 @@ def rtm_shot(p, vp, vs, shot, precision=64, rec_in=None):
     _check(lib().oracle_rtm_shot(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(rin),
                                  *[abi.dptr(a) for a in outs], abi.dptr(rec_out)), "rtm")
     return {"img_pp": outs[0], "img_ps": outs[1], "lap_pp": outs[2], "lap_ps": outs[3], "rec": rec_out}
+
+
+def forward_splitpml(p, vp, vs, shot, precision=64):
+    """Seismograms [6][nx][nt] of the first-order split-PML engine."""
+    m = _ModelRef(vp, vs)
+    s = _ShotRef(**shot)
+    out = np.zeros((6, p.nx, p.nt), np.float64)
+    _check(lib().oracle_forward_splitpml(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(out)),
+           "split-PML forward")
+    return out
 
 
 def fwi_gradient(p, vp, vs, shot, observed, precision=64):

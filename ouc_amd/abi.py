@@ -29,6 +29,7 @@ LAP_FFT5 = 1
 LAP_FFT_CONT = 2
 ENGINE_PLAIN = 0
 ENGINE_FUSED = 1
+ENGINE_SPLIT_PML = 2
 
 
 class Params(C.Structure):

@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "fused.hpp"
+#include "splitpml.hpp"
 #include "lapfft.hpp"
 
 namespace ouc {
@@ -449,6 +450,11 @@ ouc_status ouc_forward_shot(ouc_ctx *c, const ouc_shot *s, float *seis) {
             fused_forward_one(c, s, seis);
             return;
         }
+        if (c->p.engine == OUC_ENGINE_SPLIT_PML) {
+            if (s->mute) throw Error(OUC_UNSUPPORTED, "the split-PML engine records unmuted seismograms");
+            splitpml_forward(c, s, shot_wavelet(c, s), seis);
+            return;
+        }
         const Grid &g = c->g;
         std::vector<float> wav = shot_wavelet(c, s);
         c->last = ouc_timings{};
@@ -485,6 +491,7 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
+        if (c->p.engine == OUC_ENGINE_SPLIT_PML) throw Error(OUC_UNSUPPORTED, "the split-PML engine models forward only");
         if (c->p.engine == OUC_ENGINE_FUSED) {
             if (rec_in) check_full_length(c, s);
             fused_rtm_one(c, s, rec_in, img_pp, img_ps, img_pp_lap, img_ps_lap);

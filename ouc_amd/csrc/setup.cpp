@@ -17,7 +17,8 @@ std::string make_grid(const ouc_params *p, Grid &g) {
     if (p->semantics != OUC_SEM_FAITHFUL && p->semantics != OUC_SEM_EXACT) return "unknown semantics";
     if (p->strip_precision != OUC_STRIP_FP32 && p->strip_precision != OUC_STRIP_FP16) return "unknown strip precision";
     if (p->laplacian < OUC_LAP_STENCIL || p->laplacian > OUC_LAP_FFT_CONT) return "unknown laplacian";
-    if (p->engine != OUC_ENGINE_PLAIN && p->engine != OUC_ENGINE_FUSED) return "unknown engine";
+    if (p->engine != OUC_ENGINE_PLAIN && p->engine != OUC_ENGINE_FUSED && p->engine != OUC_ENGINE_SPLIT_PML)
+        return "unknown engine";
     if (p->boundary != OUC_BOUNDARY_CPML && p->boundary != OUC_BOUNDARY_LEARNED &&
         p->boundary != OUC_BOUNDARY_LEARNED_FALLBACK)
         return "unknown boundary";
