@@ -624,15 +624,37 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
 // strips; the reverse step uses theta^{t+1} recomputed from V^{t+1}.  Writes the
 // source P velocity of the new level for the imaging condition.
 // ---------------------------------------------------------------------------
-// Stress-like fields at (i, j) for the reverse step: strips on the ring,
-// computed from the reconstructed totals elsewhere.
+// Stores stress-like fields of the ring frame and interior for the reverse
+// step of the next launch (the stress is a function of V^{t+1} only).  The
+// block's 64 x 4 tile and its N-cell halo of totals (ring cells from the
+// strips) are staged in LDS once, so each total is looked up once per tile
+// rather than once per stencil tap.
 template <int N>
-__device__ __forceinline__ void recon_stress_at(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s,
-                                                int rb, long so, const float *__restrict__ p2,
-                                                const float *__restrict__ q2, int i, int j, const FStep &a,
-                                                float &th, float &om, float &gxx, float &gxz, float &gzx,
-                                                float &gzz) {
-    long r = ring_index(rc.ring, i, j);
+__global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const float *__restrict__ p2,
+                                                        const float *__restrict__ q2, FRecon rc, FStep a,
+                                                        const int2 *__restrict__ tiles) {
+    constexpr int HX = 4 + 2 * N, HZ = 64 + 2 * N;
+    __shared__ float lx[HX][HZ], lz[HX][HZ];
+    const int2 t0 = tiles[blockIdx.x];
+    const long so = (long)blockIdx.z * g.plane;
+    const KStrips s = shot_strips(rc);
+    for (int e = threadIdx.y * 64 + threadIdx.x; e < HX * HZ; e += 256) {
+        const int xi = e / HZ, zj = e - xi * HZ;
+        float vx, vz;
+        recon_vin(g, w, rc, s, a.rb, so, t0.x - N + xi, t0.y - N + zj, vx, vz);
+        lx[xi][zj] = vx;
+        lz[xi][zj] = vz;
+    }
+    __syncthreads();
+    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
+    if (i >= g.Xn - g.L + N || j >= g.Zn - g.L + N) return;
+    // only the band next to the ring and its N-cell margin: the recon core forms its own
+    if (i >= g.L + 3 * N && i < g.Xn - g.L - 3 * N && j >= g.L + 3 * N && j < g.Zn - g.L - 3 * N) return;
+    const long r = ring_index(rc.ring, i, j);
+    // outer ring cells have no interior neighbours to differentiate; the
+    // reconstruction reads them only as strip values
+    if (r < 0 && (i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L)) return;
+    float th, om, gxx, gxz, gzx, gzz;
     if (r >= 0) {
         th = strip_get(s, rc.slot_th, RS_TH, r);
         om = strip_get(s, rc.slot_th, RS_OM, r);
@@ -640,50 +662,23 @@ __device__ __forceinline__ void recon_stress_at(const FGrid &g, const FWave &w, 
         gzz = strip_get(s, rc.slot_th, RS_GZZ, r);
         gxx = strip_get(s, rc.slot_th, RS_GXX, r);
         gxz = strip_get(s, rc.slot_th, RS_GXZ, r);
-        return;
-    }
-    const float *c = g.c;
-    float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, a0, b0, a1, b1;
+    } else {
+        const float *c = g.c;
+        const int xl = threadIdx.y + N, zl = threadIdx.x + N;
+        float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
 #pragma unroll
-    for (int mm = 1; mm <= N; ++mm) {
-        recon_vin(g, w, rc, s, rb, so, i + mm - 1, j, a0, b0);
-        recon_vin(g, w, rc, s, rb, so, i - mm, j, a1, b1);
-        s1 += c[mm - 1] * (a0 - a1);
-        recon_vin(g, w, rc, s, rb, so, i + mm, j, a0, b0);
-        recon_vin(g, w, rc, s, rb, so, i - mm + 1, j, a1, b1);
-        s2 += c[mm - 1] * (b0 - b1);
-        recon_vin(g, w, rc, s, rb, so, i, j + mm, a0, b0);
-        recon_vin(g, w, rc, s, rb, so, i, j - mm + 1, a1, b1);
-        s3 += c[mm - 1] * (a0 - a1);
-        recon_vin(g, w, rc, s, rb, so, i, j + mm - 1, a0, b0);
-        recon_vin(g, w, rc, s, rb, so, i, j - mm, a1, b1);
-        s4 += c[mm - 1] * (b0 - b1);
+        for (int mm = 1; mm <= N; ++mm) {
+            s1 += c[mm - 1] * (lx[xl + mm - 1][zl] - lx[xl - mm][zl]);
+            s2 += c[mm - 1] * (lz[xl + mm][zl] - lz[xl - mm + 1][zl]);
+            s3 += c[mm - 1] * (lx[xl][zl + mm] - lx[xl][zl - mm + 1]);
+            s4 += c[mm - 1] * (lz[xl][zl + mm - 1] - lz[xl][zl - mm]);
+        }
+        gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
+        const long k0 = (long)i * g.P + j;
+        th = p2[k0] * (gxx + gzz);
+        if (rc.exact) th += src_term(g, a, i, j);
+        om = q2[k0] * (gxz - gzx);
     }
-    gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
-    const long k0 = (long)i * g.P + j;
-    th = p2[k0] * (gxx + gzz);
-    if (rc.exact) th += src_term(g, a, i, j);
-    om = q2[k0] * (gxz - gzx);
-}
-
-// Stores stress-like fields of the ring frame and interior for the reverse
-// step of the next launch (the stress is a function of V^{t+1} only).
-template <int N>
-__global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const float *__restrict__ p2,
-                                                        const float *__restrict__ q2, FRecon rc, FStep a,
-                                                        const int2 *__restrict__ tiles) {
-    const int2 t0 = tiles[blockIdx.x];
-    const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
-    if (i >= g.Xn - g.L + N || j >= g.Zn - g.L + N) return;
-    // only the band next to the ring and its N-cell margin: the recon core forms its own
-    if (i >= g.L + 3 * N && i < g.Xn - g.L - 3 * N && j >= g.L + 3 * N && j < g.Zn - g.L - 3 * N) return;
-    const long so = (long)blockIdx.z * g.plane;
-    const KStrips s = shot_strips(rc);
-    float th, om, gxx, gxz, gzx, gzz;
-    // outer ring cells have no interior neighbours to differentiate; the
-    // reconstruction reads them only as strip values
-    if (ring_index(rc.ring, i, j) < 0 && (i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L)) return;
-    recon_stress_at<N>(g, w, rc, s, a.rb, so, p2, q2, i, j, a, th, om, gxx, gxz, gzx, gzz);
     const long k = so + (long)i * g.P + j;
     w.th[k] = th; w.om[k] = om; w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
 }
