@@ -1,5 +1,5 @@
 """Learned absorbing boundary: training data from the engine, training on the
-GPU, and the reflected-energy judge.
+GPU, and the reflected-energy measure.
 
 Method (no counterpart in the reference, whose boundary is the CPML of
 P-S-seperating.cu:2067-2105):
@@ -23,7 +23,7 @@ P-S-seperating.cu:2067-2105):
   inputs and targets teaches the network to damp what it did not see in the
   training wavefields; without it the one-step fit is accurate and the rollout
   still grows.
-* The judge is the energy of the difference between the small-grid run and the
+* The measure of quality is the energy of the difference between the small-grid run and the
   extended run over the interior, relative to the extended run's energy, in dB.
 """
 import os
@@ -277,7 +277,7 @@ def train(x, y, m, epochs=3000, lr=2e-3, seed=0, device=0, w0=None):
 
 
 # ---------------------------------------------------------------------------
-# The judge: reflected energy against the extended run
+# Quality measure: reflected energy against the extended run
 # ---------------------------------------------------------------------------
 def interior_snapshots(e, case, steps, origin, pad):
     return e.forward_snapshots(case.shot(origin - pad), steps, origin, origin, case.n, case.n)
