@@ -53,9 +53,9 @@ def test_fwi_gradient_vs_oracle(oracle, observed, semantics, strips):
 
 
 def test_fwi_batch_is_sum_of_shots_and_replays(oracle, observed):
-    """Two shots in one call equal the sum of the two single-shot calls; a
-    second call on the same context (the captured backward loop replayed)
-    returns the same bits."""
+    """Two shots in one call equal the sum of the two single-shot calls, also
+    when the batch capacity splits them into two batches; a second call on the
+    same context (the captured backward loop replayed) returns the same bits."""
     p, vp0, vs0, vpt, shot = _anomaly_case(abi.SEM_FAITHFUL, nt=500)
     shot2 = dict(shot, sx=25)
     obs2 = oracle.rtm_shot(p, vpt, vs0, shot2)["rec"]
@@ -66,11 +66,15 @@ def test_fwi_batch_is_sum_of_shots_and_replays(oracle, observed):
         again = e.fwi_gradient([shot, shot2], obs)
         one = e.fwi_gradient([shot], obs[:1])
         two = e.fwi_gradient([shot2], obs[1:])
+        e.set_batch_capacity(1)  # the two shots now run as two batches of one
+        split = e.fwi_gradient([shot, shot2], obs)
     for k in KEYS:
         np.testing.assert_array_equal(both[k], again[k])
         s = one[k].astype(np.float64) + two[k]
         assert rel_l2(both[k], s) < 1e-6, k
+        assert rel_l2(split[k], s) < 1e-6, k
     assert both["misfit"] == pytest.approx(one["misfit"] + two["misfit"], rel=1e-12)
+    assert split["misfit"] == pytest.approx(both["misfit"], rel=1e-12)
 
 
 def test_model_update_bit_exact():

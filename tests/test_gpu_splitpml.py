@@ -41,3 +41,21 @@ def test_splitpml_forward_two_layer_200(oracle):
         assert errs[c] < 1e-5, (c, errs[c])
     for c in ("vpx", "vpz", "vsx", "vsz"):
         assert errs[c] < 10 * max(e32[c], 1e-6), (c, errs[c], e32[c])
+
+
+def test_splitpml_twelfth_order_coefficients(oracle):
+    """N = 6 runs with the split-PML kernels' five-digit coefficients on both
+    sides (a small case; totals at the field bar)."""
+    p = abi.make_params(64, 48, 12, 6, 300, 5.0, 5.0, 0.0005, R=1e-3)
+    z = np.arange(48)
+    vp = np.tile(np.where(z < 24, 2000.0, 2600.0), (64, 1)).astype(np.float32)
+    vs = (vp / 1.7).astype(np.float32)
+    shot = dict(sx=32, sz=2, rec_z=2, fm=30.0, mute=0)
+    ref = oracle.forward_splitpml(p, vp, vs, shot, 64)
+    p.engine = abi.ENGINE_SPLIT_PML
+    print(f"[case] split-PML N=6 64x48 nt={p.nt}", flush=True)
+    with Engine(p, vp, vs) as e:
+        seis = e.forward_shot(shot)
+    errs = [rel_l2(seis[q], ref[q]) for q in range(2)]
+    print("[split-PML N=6] vx, vz rel-L2 vs FP64:", " ".join(f"{v:.2e}" for v in errs), flush=True)
+    assert max(errs) < 1e-5, errs
