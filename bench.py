@@ -53,6 +53,9 @@ def parse():
                     help="FWI gradient throughput (shots/hour) on the same workload instead of RTM")
     ap.add_argument("--split-pml", action="store_true",
                     help="forward modelling with the first-order split-PML engine (one shot per step)")
+    ap.add_argument("--stream-probe", default="",
+                    help="NINxNOUT: streaming rate of a kernel reading NIN and writing NOUT arrays "
+                         "(4.76 GB per launch in total, the forward core launch's compulsory bytes)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -85,6 +88,15 @@ def main():
         sys.exit(2)
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib)
+        return
+    if a.stream_probe:
+        from ouc_amd.engine import stream_probe
+
+        nin, nout = (int(v) for v in a.stream_probe.lower().split("x"))
+        n = int(4.7613e9 / 4 / (nin + nout))
+        ms = stream_probe(nin, nout, n, reps=20)
+        print(json.dumps({"stream_probe": f"{nin}x{nout}", "floats_per_array": n, "bytes_per_launch": (nin + nout) * n * 4,
+                          "ms_per_launch": ms, "gbs": (nin + nout) * n * 4 / ms / 1e6}))
         return
     if a.serve:
         serve_bench(a)
