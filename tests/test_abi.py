@@ -86,3 +86,19 @@ def test_argument_errors_need_no_device():
     m = abi.Model(vp, vp, None)
     assert L.ouc_create(C.byref(p), C.byref(m), 0, C.byref(h)) == abi.OUC_INVALID_ARG
     assert b"fused" in L.ouc_last_error(None)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libouc.so not built")
+def test_argument_checks_need_no_gpu():
+    """Entry points reject bad arguments before touching a device."""
+    import numpy as np
+
+    from ouc_amd import abi
+    from ouc_amd.engine import lib
+
+    ms = np.zeros(1, np.float32)
+    assert lib().ouc_stream_probe(0, 0, 1, 10, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_stream_probe(0, 1, 17, 10, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    v = np.zeros(4, np.float32)
+    assert lib().ouc_model_update(0, 0, abi.fptr(v), abi.fptr(v), 30.0, 0, abi.fptr(v)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_fwi_gradient(None, None, 1, None, None, None, None, None) == abi.OUC_INVALID_ARG

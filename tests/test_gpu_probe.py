@@ -1,0 +1,12 @@
+"""The streaming-rate probe runs and reports a launch time."""
+import pytest
+
+from ouc_amd.engine import stream_probe
+
+pytestmark = pytest.mark.gpu
+
+
+def test_stream_probe_times_launches():
+    ms = stream_probe(10, 8, 1 << 20, reps=3)
+    print(f"[probe] 10x8 x 1M floats: {ms:.4f} ms per launch", flush=True)
+    assert 0 < ms < 100
