@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--split-pml", action="store_true",
                     help="forward modelling with the first-order split-PML engine (one shot per step)")
     ap.add_argument("--stream-probe", default="",
-                    help="NINxNOUT: streaming rate of a kernel reading NIN and writing NOUT arrays "
-                         "(4.76 GB per launch in total, the forward core launch's compulsory bytes)")
+                    help="NINxNOUT[sS]: streaming rate of a kernel reading NIN and writing NOUT arrays "
+                         "(4.76 GB per launch in total, the forward core launch's compulsory bytes); with "
+                         "sS the last S inputs are one 2048x1024 model plane read by every shot")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -92,10 +93,14 @@ def main():
     if a.stream_probe:
         from ouc_amd.engine import stream_probe
 
-        nin, nout = (int(v) for v in a.stream_probe.lower().split("x"))
+        spec = a.stream_probe.lower()
+        nshared = int(spec.split("s")[1]) if "s" in spec else 0
+        nin, nout = (int(v) for v in spec.split("s")[0].split("x"))
         n = int(4.7613e9 / 4 / (nin + nout))
-        ms = stream_probe(nin, nout, n, reps=20)
-        print(json.dumps({"stream_probe": f"{nin}x{nout}", "floats_per_array": n, "bytes_per_launch": (nin + nout) * n * 4,
+        plane = 2108 * 1088  # one pitched 2048x1024 plane with its 30-cell pad, floats
+        ms = stream_probe(nin, nout, n, reps=20, nshared=nshared, m=plane if nshared else 0)
+        print(json.dumps({"stream_probe": spec, "floats_per_array": n, "shared_inputs": nshared,
+                          "shared_floats": plane if nshared else 0, "bytes_per_launch": (nin + nout) * n * 4,
                           "ms_per_launch": ms, "gbs": (nin + nout) * n * 4 / ms / 1e6}))
         return
     if a.serve:

@@ -224,9 +224,11 @@ ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best
  * times `reps` launches of a kernel in which every thread reads nin arrays
  * and writes nout arrays of n floats at its own index (1..16 each), the
  * access pattern of the core kernel's state streams, and returns the mean
- * launch time.  Device memory is held for the call only. */
-ouc_status ouc_stream_probe(int32_t device, int32_t nin, int32_t nout, int64_t n, int32_t reps,
-                            float *ms_per_launch);
+ * launch time.  The last nshared (< nin) inputs hold only m floats and are
+ * read at index mod m, as the core kernel reads the model plane that all shots
+ * share.  Device memory is held for the call only. */
+ouc_status ouc_stream_probe(int32_t device, int32_t nin, int32_t nout, int64_t n, int32_t nshared, int64_t m,
+                            int32_t reps, float *ms_per_launch);
 
 /* gfx architecture name of a HIP device (e.g. "gfx950"), for tuning caches. */
 ouc_status ouc_device_arch(int32_t device, char *buf, int32_t len);
