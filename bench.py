@@ -54,9 +54,11 @@ def parse():
     ap.add_argument("--split-pml", action="store_true",
                     help="forward modelling with the first-order split-PML engine (one shot per step)")
     ap.add_argument("--stream-probe", default="",
-                    help="NINxNOUT[sS]: streaming rate of a kernel reading NIN and writing NOUT arrays "
+                    help="NINxNOUT[sS][wW]: streaming rate of a kernel reading NIN and writing NOUT arrays "
                          "(4.76 GB per launch in total, the forward core launch's compulsory bytes); with "
-                         "sS the last S inputs are one 2048x1024 model plane read by every shot")
+                         "sS the last S inputs are one 2048x1024 model plane read by every shot; wW moves W "
+                         "consecutive floats per thread (1, 2, 4; default 1; -2: float2 memory ops transposed "
+                         "through LDS, one value per thread for the arithmetic)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -93,13 +95,18 @@ def main():
     if a.stream_probe:
         from ouc_amd.engine import stream_probe
 
+        import re
+
         spec = a.stream_probe.lower()
-        nshared = int(spec.split("s")[1]) if "s" in spec else 0
-        nin, nout = (int(v) for v in spec.split("s")[0].split("x"))
-        n = int(4.7613e9 / 4 / (nin + nout))
+        mt = re.fullmatch(r"(\d+)x(\d+)(?:s(\d+))?(?:w(-?\d+))?", spec)
+        if not mt:
+            raise SystemExit(f"bad --stream-probe {spec}")
+        nin, nout = int(mt.group(1)), int(mt.group(2))
+        nshared, width = int(mt.group(3) or 0), int(mt.group(4) or 1)
+        n = int(4.7613e9 / 4 / (nin + nout)) // 256 * 256
         plane = 2108 * 1088  # one pitched 2048x1024 plane with its 30-cell pad, floats
-        ms = stream_probe(nin, nout, n, reps=20, nshared=nshared, m=plane if nshared else 0)
-        print(json.dumps({"stream_probe": spec, "floats_per_array": n, "shared_inputs": nshared,
+        ms = stream_probe(nin, nout, n, reps=20, nshared=nshared, m=plane if nshared else 0, width=width)
+        print(json.dumps({"stream_probe": spec, "width": width, "floats_per_array": n, "shared_inputs": nshared,
                           "shared_floats": plane if nshared else 0, "bytes_per_launch": (nin + nout) * n * 4,
                           "ms_per_launch": ms, "gbs": (nin + nout) * n * 4 / ms / 1e6}))
         return

@@ -226,9 +226,15 @@ ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best
  * access pattern of the core kernel's state streams, and returns the mean
  * launch time.  The last nshared (< nin) inputs hold only m floats and are
  * read at index mod m, as the core kernel reads the model plane that all shots
- * share.  Device memory is held for the call only. */
+ * share.  width (1, 2 or 4) is the number of consecutive floats each thread
+ * moves per array (1: the core kernel's one cell per lane); width -2 keeps one
+ * value per thread for the arithmetic and moves 256-float chunks as float2
+ * loads and stores transposed through LDS, width -3 moves two arrays per
+ * float2 instruction within a wave and redistributes them with cross-lane
+ * permutes (no shared inputs for either).  n is a multiple of 256 and m of 4.
+ * Device memory is held for the call only. */
 ouc_status ouc_stream_probe(int32_t device, int32_t nin, int32_t nout, int64_t n, int32_t nshared, int64_t m,
-                            int32_t reps, float *ms_per_launch);
+                            int32_t width, int32_t reps, float *ms_per_launch);
 
 /* gfx architecture name of a HIP device (e.g. "gfx950"), for tuning caches. */
 ouc_status ouc_device_arch(int32_t device, char *buf, int32_t len);

@@ -55,7 +55,8 @@ def lib():
     L.ouc_write_segy.restype = C.c_int
     L.ouc_read_segy.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), fp, fp, C.c_int64]
     L.ouc_read_segy.restype = C.c_int
-    L.ouc_stream_probe.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, fp]
+    L.ouc_stream_probe.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32,
+                                   C.c_int32, fp]
     L.ouc_stream_probe.restype = C.c_int
     L.ouc_fwi_gradient.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, fp, fp, fp, fp, P(C.c_double)]
     L.ouc_fwi_gradient.restype = C.c_int
@@ -345,13 +346,13 @@ def read_segy(path):
     return out, dt.value
 
 
-def stream_probe(nin, nout, n, reps=10, nshared=0, m=0, device=0):
+def stream_probe(nin, nout, n, reps=10, nshared=0, m=0, width=1, device=0):
     """Mean launch time (ms) of a kernel reading nin and writing nout arrays of
-    n floats, one element per thread; the last nshared inputs hold m floats,
-    read at index mod m."""
+    n floats, `width` consecutive floats per thread; the last nshared inputs
+    hold m floats, read at index mod m."""
     ms = np.zeros(1, np.float32)
-    _io_check(lib().ouc_stream_probe(int(device), int(nin), int(nout), int(n), int(nshared), int(m), int(reps),
-                                     abi.fptr(ms)))
+    _io_check(lib().ouc_stream_probe(int(device), int(nin), int(nout), int(n), int(nshared), int(m), int(width),
+                                     int(reps), abi.fptr(ms)))
     return float(ms[0])
 
 

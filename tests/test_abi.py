@@ -97,9 +97,10 @@ def test_argument_checks_need_no_gpu():
     from ouc_amd.engine import lib
 
     ms = np.zeros(1, np.float32)
-    assert lib().ouc_stream_probe(0, 0, 1, 10, 0, 0, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
-    assert lib().ouc_stream_probe(0, 1, 17, 10, 0, 0, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
-    assert lib().ouc_stream_probe(0, 2, 1, 10, 2, 5, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_stream_probe(0, 0, 1, 10, 0, 0, 1, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_stream_probe(0, 1, 17, 10, 0, 0, 1, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_stream_probe(0, 2, 1, 10, 2, 5, 1, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
+    assert lib().ouc_stream_probe(0, 2, 1, 10, 0, 0, 3, 1, abi.fptr(ms)) == abi.OUC_INVALID_ARG
     v = np.zeros(4, np.float32)
     assert lib().ouc_model_update(0, 0, abi.fptr(v), abi.fptr(v), 30.0, 0, abi.fptr(v)) == abi.OUC_INVALID_ARG
     assert lib().ouc_fwi_gradient(None, None, 1, None, None, None, None, None) == abi.OUC_INVALID_ARG

@@ -15,3 +15,5 @@ def test_stream_probe_times_launches():
 def test_stream_probe_with_shared_inputs():
     ms = stream_probe(10, 8, 1 << 20, reps=3, nshared=2, m=1 << 16)
     assert 0 < ms < 100
+    for w in (2, 4):
+        assert 0 < stream_probe(10, 8, 1 << 20, reps=3, width=w) < 100
