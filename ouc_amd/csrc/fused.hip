@@ -707,8 +707,9 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         const int lev = exact ? t + 1 : std::min(t + 2, nt);
         r.t = t;
         r.rb = rbr;
-        r.spx = st->vp[lev % 3][0].f();
-        r.spz = st->vp[lev % 3][1].f();
+        // the FWI gradient needs no image: its receiver step runs without the imaging condition
+        r.spx = fwi ? nullptr : st->vp[lev % 3][0].f();
+        r.spz = fwi ? nullptr : st->vp[lev % 3][1].f();
         CoreProbe none;
         int nl2 = 0;
         wave_step_probed(c, st, L, st->R, r, none, nl2, mon_r);
