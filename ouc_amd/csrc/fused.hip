@@ -683,8 +683,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             rc.slot_v = handoff ? -1 : (exact ? t : t + 1);
             rc.slot_th = t + 1;
             rc.exact = (exact || handoff) ? 1 : 0;
-            rc.vpx = st->vp[t % 3][0].f();
-            rc.vpz = st->vp[t % 3][1].f();
+            rc.vpx = fwi ? nullptr : st->vp[t % 3][0].f();
+            rc.vpz = fwi ? nullptr : st->vp[t % 3][1].f();
             FStep ra{};
             ra.t = t + 1;
             ra.rb = rbs;

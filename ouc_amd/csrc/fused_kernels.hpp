@@ -158,7 +158,7 @@ struct FRecon {
     int slot_v;          // slot of the ring totals (< 0: the state's own, no override)
     int slot_th;         // slot of the ring stress-like fields
     int exact;           // add the source term (EXACT semantics, or the hand-off step)
-    float *vpx, *vpz;    // [shot][plane] output level
+    float *vpx, *vpz;    // [shot][plane] output level, or NULL (no imaging condition)
 };
 
 __device__ __forceinline__ long ring_index(const KRing &r, int i, int j) {
@@ -431,8 +431,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             d = st6[4] + sg * dt2 * (hx + Kx); stb(DS0, k, d); const float sx = st6[2] + sg * d; stb(S0, k, sx);
             d = st6[5] + sg * dt2 * (hz + Kz); stb(DS1, k, d); const float sz = st6[3] + sg * d; stb(S1, k, sz);
             if (RECON) {
-                stb(rc.vpx + so, k, vx - sx);
-                stb(rc.vpz + so, k, vz - sz);
+                if (rc.vpx) { // the source P level for the imaging condition (none for the FWI gradient)
+                    stb(rc.vpx + so, k, vx - sx);
+                    stb(rc.vpz + so, k, vz - sz);
+                }
             } else {
                 record(g, a, cu, z, vx, vz, sx, sz);
                 if (IMG && img_on) { // corr_v (P-S-seperating.cu:1426-1439); core cells are interior
@@ -722,8 +724,10 @@ __global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, cons
     w.V[wb][1][k] = vz;
     d = w.DS[0][k] - dt2 * (Wz + KX); w.DS[0][k] = d; const float sx = w.S[0][k] - d; w.S[0][k] = sx;
     d = w.DS[1][k] - dt2 * (-Wx + KZ); w.DS[1][k] = d; const float sz = w.S[1][k] - d; w.S[1][k] = sz;
-    rc.vpx[k] = vx - sx;
-    rc.vpz[k] = vz - sz;
+    if (rc.vpx) {
+        rc.vpx[k] = vx - sx;
+        rc.vpz[k] = vz - sz;
+    }
 }
 
 // Strip store from the stored band stress and the written V (forward).
