@@ -3,17 +3,18 @@
 A checker run by hand, in three phases (the FP64 oracle takes minutes on the
 host cores and the GPU box has no copy of its output):
 
-  python tests/config_fwi_error.py oracle  --out DIR   # CPU: observed records and FP64 gradient (.npy)
-  python tests/config_fwi_error.py gpu     --out DIR --ref DIR [--strips fp16|fp32]
-                                                       # GPU: gradient from the same observed records
-  python tests/config_fwi_error.py compare --ref DIR --gpu DIR
+  python tests/config_fwi_error.py oracle  --out DIR [--config C]   # CPU: observed records and FP64 gradient (.npy)
+  python tests/config_fwi_error.py gpu     --out DIR --ref DIR [--config C] [--strips fp16|fp32]
+                                                                    # GPU: gradient from the same observed records
+  python tests/config_fwi_error.py compare --ref DIR --gpu DIR [--config C]
 
-Workload: the 1024 x 512 RTM configuration (six layers 1800-4000 m/s with a
-3 % perturbation, N = 5, 30-cell CPML, Ricker 20 Hz, dt 0.4 ms, 4000 steps,
-one shot at x = 512 on the surface).  The observed records come from the
-same model with a +5 % vp lens (radius 40 cells at x = 512, z = 300); the
-gradient is taken in the unperturbed model, so the residual is the lens's
-scattered wavefield.  Errors are relative L2 over the interior below row 8 and
+Workloads (seeded layered models with a 3 % perturbation, N = 5, 30-cell
+CPML, Ricker 20 Hz, dt 0.4 ms, one surface shot):
+  rtm1024: 1024 x 512, 4000 steps, shot at x = 512;
+  rtm2048: 2048 x 1024, 6000 steps, shot 32 of the 64-shot layout (x = 1040).
+The observed records come from the same model with a +5 % vp lens (radius
+40 cells, below the shot at z = 300); the gradient is taken in the
+unperturbed model, so the residual is the lens's scattered wavefield.  Errors are relative L2 over the interior below row 8 and
 in bands of horizontal distance from the source, as for the images.
 """
 import argparse
@@ -31,22 +32,26 @@ KEYS = ("grad_vp", "grad_vs", "illum")
 BANDS = ((0, 128), (128, 256), (256, 512))
 
 
-def workload():
+def workload(config="rtm1024"):
     from ouc_amd import velocity_models as vm
 
-    p, vp, vs, shot = vm.cfg_rtm_1024()
+    if config == "rtm1024":
+        p, vp, vs, shot = vm.cfg_rtm_1024()
+    else:
+        p, vp, vs, shots = vm.cfg_rtm_2048()
+        shot = shots[32]
     shot = dict(shot, mute=0)
     x, z = np.meshgrid(np.arange(p.nx), np.arange(p.nz), indexing="ij")
-    lens = ((x - 512) ** 2 + (z - 300) ** 2) < 40 ** 2
+    lens = ((x - shot["sx"]) ** 2 + (z - 300) ** 2) < 40 ** 2
     vpt = np.where(lens, vp * np.float32(1.05), vp).astype(np.float32)
     return p, vp, vs, vpt, shot
 
 
-def run_oracle(out):
+def run_oracle(out, config):
     from oracle import pyoracle
 
     pyoracle.set_threads(os.cpu_count() or 1)
-    p, vp, vs, vpt, shot = workload()
+    p, vp, vs, vpt, shot = workload(config)
     t0 = time.time()
     obs = pyoracle.rtm_shot(p, vpt, vs, shot)["rec"]
     np.save(os.path.join(out, "observed.npy"), obs.astype(np.float32))
@@ -55,18 +60,18 @@ def run_oracle(out):
     for k in KEYS:
         np.save(os.path.join(out, f"{k}.npy"), r[k])
     json.dump({"misfit": r["misfit"]}, open(os.path.join(out, "misfit.json"), "w"))
-    print(f"[oracle FP64] fwi 1024x512: {time.time() - t0:.0f} s", flush=True)
+    print(f"[oracle FP64] fwi {config}: {time.time() - t0:.0f} s", flush=True)
 
 
-def run_gpu(out, ref, strips):
+def run_gpu(out, ref, strips, config):
     from ouc_amd import abi
     from ouc_amd.engine import Engine
 
-    p, vp, vs, vpt, shot = workload()
+    p, vp, vs, vpt, shot = workload(config)
     p.engine = abi.ENGINE_FUSED
     p.strip_precision = abi.STRIP_FP16 if strips == "fp16" else abi.STRIP_FP32
     obs = np.load(os.path.join(ref, "observed.npy"))
-    print(f"[gpu] fwi 1024x512, {strips} strips", flush=True)
+    print(f"[gpu] fwi {config}, {strips} strips", flush=True)
     with Engine(p, vp, vs) as e:
         r = e.fwi_gradient([shot], obs[None])
     for k in KEYS:
@@ -78,8 +83,8 @@ def rel(a, b, m):
     return float(np.linalg.norm((a - b)[m]) / np.linalg.norm(b[m]))
 
 
-def compare(ref, gpu):
-    _, _, _, _, shot = workload()
+def compare(ref, gpu, config):
+    _, _, _, _, shot = workload(config)
     res = {}
     for k in KEYS:
         a = np.load(os.path.join(gpu, f"{k}.npy")).astype(np.float64)
@@ -105,15 +110,16 @@ def main():
     ap.add_argument("--ref", default="")
     ap.add_argument("--gpu", default="")
     ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--config", choices=["rtm1024", "rtm2048"], default="rtm1024")
     a = ap.parse_args()
     if a.phase == "oracle":
         os.makedirs(a.out, exist_ok=True)
-        run_oracle(a.out)
+        run_oracle(a.out, a.config)
     elif a.phase == "gpu":
         os.makedirs(a.out, exist_ok=True)
-        run_gpu(a.out, a.ref, a.strips)
+        run_gpu(a.out, a.ref, a.strips, a.config)
     else:
-        compare(a.ref, a.gpu)
+        compare(a.ref, a.gpu, a.config)
 
 
 if __name__ == "__main__":
