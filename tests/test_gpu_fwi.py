@@ -84,3 +84,22 @@ def test_model_update_bit_exact():
     for it in (0, 1, 7):
         got = model_update(v, g, 30.0, it)
         np.testing.assert_array_equal(got, model_update_ref(v, g, 30.0, it))
+
+
+def test_fwi_refuses_what_it_does_not_run(observed):
+    """The gradient runs on the fused engine with CPML and full-length shots;
+    other requests return OUC_UNSUPPORTED instead of a wrong answer."""
+    from ouc_amd.engine import OucError
+
+    p, vp0, vs0, vpt, shot = _anomaly_case(abi.SEM_FAITHFUL, nt=500)
+    obs = observed[abi.SEM_FAITHFUL][None]
+    p.engine = abi.ENGINE_PLAIN
+    with Engine(p, vp0, vs0) as e:
+        with pytest.raises(OucError) as err:
+            e.fwi_gradient([shot], obs)
+        assert err.value.status == abi.OUC_UNSUPPORTED
+    p.engine = abi.ENGINE_FUSED
+    with Engine(p, vp0, vs0) as e:
+        with pytest.raises(OucError) as err:
+            e.fwi_gradient([dict(shot, nt=400)], obs)
+        assert err.value.status == abi.OUC_UNSUPPORTED

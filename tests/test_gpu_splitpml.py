@@ -59,3 +59,13 @@ def test_splitpml_twelfth_order_coefficients(oracle):
     errs = [rel_l2(seis[q], ref[q]) for q in range(2)]
     print("[split-PML N=6] vx, vz rel-L2 vs FP64:", " ".join(f"{v:.2e}" for v in errs), flush=True)
     assert max(errs) < 1e-5, errs
+
+
+def test_splitpml_refuses_the_mute():
+    p = abi.make_params(64, 48, 12, 5, 50, 5.0, 5.0, 0.0005, R=1e-3, engine=abi.ENGINE_SPLIT_PML)
+    vp = np.full((64, 48), 2000.0, np.float32)
+    vs = np.full((64, 48), 1100.0, np.float32)
+    with Engine(p, vp, vs) as e:
+        with pytest.raises(OucError) as err:
+            e.forward_shot(dict(sx=32, sz=2, rec_z=2, fm=30.0, mute=1))
+        assert err.value.status == abi.OUC_UNSUPPORTED
