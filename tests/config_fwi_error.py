@@ -53,7 +53,7 @@ def run_oracle(out, config):
     pyoracle.set_threads(os.cpu_count() or 1)
     p, vp, vs, vpt, shot = workload(config)
     t0 = time.time()
-    obs = pyoracle.rtm_shot(p, vpt, vs, shot)["rec"]
+    obs = pyoracle.forward_shot(p, vpt, vs, shot)[:2]  # vx, vz records (no mute)
     np.save(os.path.join(out, "observed.npy"), obs.astype(np.float32))
     obs32 = obs.astype(np.float32).astype(np.float64)  # the GPU receives float32 records
     r = pyoracle.fwi_gradient(p, vp, vs, shot, obs32)
