@@ -601,9 +601,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             FWave ws = st->S; // the stress memory variables of this step (the streaming band kernel double-buffers them)
             if (st->edge && rb == 0)
                 for (int q = 0; q < 4; ++q) ws.F[q] = st->S.F2[q];
-            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, ws, kr, ks0, sstride, t, 0, 0);
-            if (t < nt - 1)
-                k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, st->S, kr, ks0, sstride, t, 1 - rb, 1);
+            // stress-like fields of step t, and (but the last step) V^{t+1}, in one launch
+            k_fstrip_save<<<dim3(rblocks, 1, B), 256, 0, s>>>(f, ws, kr, ks0, sstride, t, 1 - rb, t < nt - 1 ? 2 : 0);
         }
         rb = 1 - rb;
     }

@@ -730,7 +730,8 @@ __global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, cons
     }
 }
 
-// Strip store from the stored band stress and the written V (forward).
+// Strip store from the stored band stress and the written V (forward):
+// with_v 0 stores the stress-like fields, 1 the velocities, 2 both.
 __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, KStripStride d, int slot,
                                                      int vbuf, int with_v) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,7 +744,8 @@ __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, 
     if (with_v) {
         strip_put(s, slot, RS_VX, q, valid, valid ? w.V[vbuf][0][k] : 0.f);
         strip_put(s, slot, RS_VZ, q, valid, valid ? w.V[vbuf][1][k] : 0.f);
-    } else {
+    }
+    if (with_v != 1) {
         strip_put(s, slot, RS_TH, q, valid, valid ? w.th[k] : 0.f);
         strip_put(s, slot, RS_OM, q, valid, valid ? w.om[k] : 0.f);
         strip_put(s, slot, RS_GZX, q, valid, valid ? w.gzx[k] - w.F[3][k] : 0.f);
