@@ -53,13 +53,13 @@ void launch_core_mode(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FW
 
 // RECON = 1: the reconstruction; otherwise a wavefield step, with the imaging
 // condition when the step carries imaging operands (a.spx).  The tuned
-// prefetch depth applies to the plain step; the reconstruction and imaging
-// modes carry more operands per column and run with one column in flight
-// (rotating queues, no register spills).
+// prefetch depth applies to the plain step; the imaging mode carries more
+// operands per column and runs with one column in flight (rotating queues, no
+// register spills), the reconstruction with two (the deepest without spills).
 template <int N, int RECON>
 void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2, const float *q2,
                  const FStep &a, const FRecon &rc) {
-    if constexpr (RECON) launch_core_mode<N, CORE_RECON>(1, grid, s, f, w, p2, q2, a, rc);
+    if constexpr (RECON) launch_core_mode<N, CORE_RECON>(2, grid, s, f, w, p2, q2, a, rc);
     else if (a.spx) launch_core_mode<N, CORE_IMAGE>(1, grid, s, f, w, p2, q2, a, rc);
     else launch_core_mode<N, CORE_STEP>(pf, grid, s, f, w, p2, q2, a, rc);
 }
