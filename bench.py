@@ -4,11 +4,13 @@ Workload: 2048 x 1024 interior at 5 m (N = 5, 30-cell CPML, 2108 x 1084 padded),
 six layers 1800-4000 m/s with a seeded 3 % random perturbation, Ricker 20 Hz,
 dt = 0.4 ms, 6000 steps per shot, 2048 surface receivers, sources at
 x = 16 + 32 s, boundary strips in scaled FP16 (FP32 compute).  A "step" is one
-batch of --batch shots (32 by default: the 64-shot layout runs as two such
-batches, the most that fit in HBM) through the full RTM path (forward with
-strip saving, mute, backward with reconstruction, receiver propagation and
-imaging, Laplacian, stack).  Inputs are resident on the GPU before the timed
-region; value = shots per hour of the whole job.
+batch of --batch shots (8 by default: the 64-shot layout runs as eight such
+batches) through the full RTM path (forward with strip saving, mute, backward
+with reconstruction, receiver propagation and imaging, Laplacian, stack).
+Inputs are resident on the GPU before the timed region; value = shots per
+hour of the whole job.  A wall-clock guard (--budget-s) stops the warmup and
+timed loops early when the next step and the legs after it would not fit, and
+the line reports the steps that ran.
 
 Also reported: Gcell-updates/s, canonical and own-bytes HBM rates, the plain
 (reference kernel structure, one shot at a time, FP32 strips) shots/hour on a
@@ -27,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
 
 
 def parse():
@@ -35,12 +38,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0,
-                    help="shots per step (0: 32 fused with FP16 strips, 16 fused with FP32 strips, 1 plain)")
+                    help="shots per step (0: 8 fused, 1 plain)")
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--engine", choices=["fused", "plain"], default="fused")
     ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp16")
     ap.add_argument("--plain-sample-shots", type=int, default=1)
-    ap.add_argument("--cpu-sample-steps", type=int, default=400)
+    ap.add_argument("--cpu-sample-s", type=float, default=10.0,
+                    help="seconds of CPU work per sampled shot of the CPU baseline (two shots)")
+    ap.add_argument("--budget-s", type=float, default=540.0,
+                    help="wall-clock budget of the whole command: steps stop early (and the line says how many "
+                         "ran) so that the line prints within it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile", default="", help="core tile COLSxPREFETCH (e.g. 256x1) instead of the autotuner's")
@@ -119,18 +126,27 @@ def main():
     if a.split_pml:
         splitpml_bench(a)
         return
-    from ouc_amd import abi, velocity_models as vm
-    from ouc_amd.engine import Engine, OucError
+    rtm_bench(a)
+
+
+def _elapsed():
+    return time.perf_counter() - T_START
+
+
+def rtm_bench(a):
+    from ouc_amd import abi, bytes_model, velocity_models as vm
+    from ouc_amd.engine import Engine
 
     engine = abi.ENGINE_FUSED if a.engine == "fused" else abi.ENGINE_PLAIN
     strips = abi.STRIP_FP16 if a.strips == "fp16" else abi.STRIP_FP32
     p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=engine, strip_precision=strips)
-    batch = a.batch or ((32 if strips == abi.STRIP_FP16 else 16) if engine == abi.ENGINE_FUSED else 1)
+    batch = a.batch or (8 if engine == abi.ENGINE_FUSED else 1)
     nsteps_total = a.warmup + a.steps
     if nsteps_total * batch > len(shots):
         shots = shots * ((nsteps_total * batch) // len(shots) + 1)
-    print(f"[bench] {a.engine} engine, batch {batch}, {a.steps} timed + {a.warmup} warmup steps, "
-          f"grid {p.nx}x{p.nz} nt={p.nt}", file=sys.stderr, flush=True)
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    log(f"{a.engine} engine, batch {batch}, {a.steps} timed + {a.warmup} warmup steps, grid {p.nx}x{p.nz} "
+        f"nt={p.nt}, budget {a.budget_s:.0f} s ({_elapsed():.1f} s used)")
     e = Engine(p, vp, vs)
     tile = None
     if engine == abi.ENGINE_FUSED:
@@ -144,23 +160,41 @@ def main():
             cols, pf, cached = tuning.tuned(e, batch)
             src = "cache" if cached else "autotuned"
         tile = {"core_cols": cols, "prefetch": pf, "source": src}
-        print(f"[bench] tile {cols} columns, prefetch {pf} ({src})", file=sys.stderr, flush=True)
+        log(f"tile {cols} columns, prefetch {pf} ({src})")
     e.set_profiling(True)
+    # The wall-clock guard keeps the whole command inside its budget: a step
+    # starts only if it, the plain and CPU legs still to come, and a margin fit.
+    reserve = (0.0 if a.no_plain or engine != abi.ENGINE_FUSED else 25.0) + \
+              (0.0 if a.no_cpu else 2 * a.cpu_sample_s + 15.0) + 20.0
     k = 0
+    step_wall = []
+    warm_done = 0
     for w in range(a.warmup):
+        if w > 0 and _elapsed() + max(step_wall) + reserve > a.budget_s:
+            log(f"budget: warmup stopped after {w} steps")
+            break
+        t1 = time.perf_counter()
         e.rtm_batch(shots[k:k + batch])
+        step_wall.append(time.perf_counter() - t1)
         k += batch
-        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+        warm_done += 1
+        log(f"warmup step {w} done ({step_wall[-1]:.1f} s, {_elapsed():.0f} s used)")
     tim = []
     t0 = time.perf_counter()
-    for s in range(a.steps):
+    for s_ in range(a.steps):
+        if tim and _elapsed() + max(step_wall) + reserve > a.budget_s:
+            log(f"budget: timed loop stopped after {len(tim)} of {a.steps} steps")
+            break
+        t1 = time.perf_counter()
         e.rtm_batch(shots[k:k + batch])
+        step_wall.append(time.perf_counter() - t1)
         k += batch
         tim.append(e.timings())
-        print(f"[bench] step {s}: {tim[-1]['total_ms']:.1f} ms device", file=sys.stderr, flush=True)
+        log(f"step {s_}: {tim[-1]['total_ms']:.1f} ms device ({_elapsed():.0f} s used)")
     wall = time.perf_counter() - t0
+    steps_run = len(tim)
     dev_ms = sum(t["total_ms"] for t in tim)
-    nshots = a.steps * batch
+    nshots = steps_run * batch
     shots_per_hour = nshots / (dev_ms / 1e3) * 3600.0
     cell_updates = sum(t["cell_updates"] for t in tim)
     canonical = sum(t["canonical_bytes"] for t in tim)
@@ -168,9 +202,7 @@ def main():
     kms = sum(t["step_kernel_ms"] for t in tim)
     klaunch = sum(t["step_kernel_launches"] for t in tim)
 
-    # dominant kernel roofline: own compulsory bytes per launch / mean launch time
-    from ouc_amd import bytes_model
-
+    # dominant kernel roofline: algorithmic bytes per launch / mean launch time (HIP events)
     per_launch_bytes, kernel_name = bytes_model.dominant_kernel_bytes(p, engine, batch)
     mean_ms = kms / max(klaunch, 1)
     achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if klaunch else None
@@ -182,35 +214,48 @@ def main():
     traffic = bytes_model.measured_traffic(kernel_name, batch if engine == abi.ENGINE_FUSED else 1)
     if traffic is not None:
         roofline["traffic"] = traffic
+    loops = None
+    if engine == abi.ENGINE_FUSED and steps_run:
+        fwd_b, bwd_b = bytes_model.fused_loop_bytes(p, batch, strips == abi.STRIP_FP16)
+        fms = sum(t["forward_ms"] for t in tim) / steps_run
+        bms = sum(t["backward_ms"] for t in tim) / steps_run
+        loops = {
+            "forward": {"bytes": fwd_b, "ms": fms, "achieved": fwd_b / (fms * 1e-3) / 1e9,
+                        "frac": fwd_b / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "backward": {"bytes": bwd_b, "ms": bms, "achieved": bwd_b / (bms * 1e-3) / 1e9,
+                         "frac": bwd_b / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "unit": "GB/s",
+            "note": "algorithmic bytes of every kernel of the loop (one batch) over the loop's wall time",
+        }
 
     plain = None
+    e.close()
     if not a.no_plain and engine == abi.ENGINE_FUSED:
         pp, _, _, _ = vm.cfg_rtm_2048(nt=a.nt, engine=abi.ENGINE_PLAIN, strip_precision=abi.STRIP_FP32)
-        e.close()
         ep = Engine(pp, vp, vs)
         ep.rtm_shot(shots[0])  # warm
         ms = 0.0
-        for s in range(a.plain_sample_shots):
-            ep.rtm_shot(shots[s])
+        for s_ in range(a.plain_sample_shots):
+            ep.rtm_shot(shots[s_])
             ms += ep.timings()["total_ms"]
         plain = {"shots_per_hour": a.plain_sample_shots / (ms / 1e3) * 3600.0,
                  "sample": f"{a.plain_sample_shots} shot(s), reference kernel structure, FP32 strips"}
         ep.close()
-    else:
-        e.close()
+        log(f"plain leg done ({_elapsed():.0f} s used)")
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(p, vp, vs, shots[0], a.cpu_sample_steps)
+        cpu = cpu_baseline(p, vp, vs, [shots[0], shots[32 % len(shots)]], a.cpu_sample_s)
+        log(f"CPU leg done ({_elapsed():.0f} s used)")
 
     line = {
         "metric": "RTM shots/hour at 2048x1024 (batched); Gcell-updates/s; HBM GB/s",
         "value": shots_per_hour,
         "unit": "shots/hour",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": dev_ms / a.steps,
+        "steps": steps_run,
+        "warmup": warm_done,
+        "ms_per_step": dev_ms / max(steps_run, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -219,16 +264,21 @@ def main():
         "config": {"workload": "batched elastic RTM, 2048x1024 interior, 5 m, N=5, 30-cell CPML, "
                                f"{p.nt} steps, {batch} shots per step, {a.strips} strips, {a.engine} engine",
                    "nx": p.nx, "nz": p.nz, "nt": p.nt, "batch": batch, "tile": tile},
+        "steps_requested": a.steps,
+        "warmup_requested": a.warmup,
         "gcell_updates_per_s": cell_updates / (dev_ms / 1e3) / 1e9,
         "canonical_gbs": canonical / (dev_ms / 1e3) / 1e9,
         "own_bytes_gbs": own / (dev_ms / 1e3) / 1e9 if own else None,
         "wall_s": wall,
+        "host_wall_ms_per_step": wall * 1e3 / max(steps_run, 1),
         "plain": plain,
         "batched_over_plain": (shots_per_hour / plain["shots_per_hour"]) if plain else None,
         "roofline": roofline,
+        "roofline_loops": loops,
         "cpu_baseline": cpu,
+        "command_s": _elapsed(),
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
 
 
 def fwi_bench(a):
@@ -321,22 +371,48 @@ def serve_bench(a):
     print(json.dumps(line))
 
 
-def cpu_baseline(p, vp, vs, shot, nsteps):
-    """FP32 oracle (OpenMP) on a bounded sample: one shot at the full grid but
-    `nsteps` time steps, extrapolated to 6000 steps."""
+def cpu_threads():
+    """Threads for the CPU baseline: the cores this process may run on, capped
+    by OMP_NUM_THREADS where the environment sets the process's CPU share."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(p, vp, vs, sample_shots, seconds):
+    """FP32 oracle (OpenMP over x columns, OMP_PROC_BIND=close) on bounded
+    samples: each listed shot at the full grid for a number of time steps sized
+    to about `seconds` of CPU time, extrapolated linearly in steps to the full
+    step count; shots/hour from the mean extrapolated shot time."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle import pyoracle
     from ouc_amd import abi
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     pyoracle.set_threads(threads)
-    ps = abi.make_params(p.nx, p.nz, p.npml, p.half_order, nsteps, p.dx, p.dz, p.dt, R=p.R)
+    # probe: a short run sizes the samples
+    probe_steps = 40
+    ps = abi.make_params(p.nx, p.nz, p.npml, p.half_order, probe_steps, p.dx, p.dz, p.dt, R=p.R)
     t0 = time.perf_counter()
-    pyoracle.rtm_shot(ps, vp, vs, shot, precision=32)
-    dt = time.perf_counter() - t0
-    per_shot_s = dt * p.nt / nsteps
+    pyoracle.rtm_shot(ps, vp, vs, sample_shots[0], precision=32)
+    per_step = (time.perf_counter() - t0) / probe_steps
+    nsteps = int(min(p.nt, max(50, seconds / max(per_step, 1e-6))))
+    ps = abi.make_params(p.nx, p.nz, p.npml, p.half_order, nsteps, p.dx, p.dz, p.dt, R=p.R)
+    shot_s = []
+    for sh in sample_shots:
+        t0 = time.perf_counter()
+        pyoracle.rtm_shot(ps, vp, vs, sh, precision=32)
+        shot_s.append((time.perf_counter() - t0) * p.nt / nsteps)
+    per_shot_s = sum(shot_s) / len(shot_s)
     return {"value": 3600.0 / per_shot_s, "unit": "shots/hour", "cores": threads, "kind": "port",
-            "sample": f"1 shot of {p.nx}x{p.nz} with {nsteps} of {p.nt} steps, FP32 oracle, "
-                      f"{dt:.1f} s, extrapolated linearly in steps"}
+            "host_cpus": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "extrapolated": True,
+            "sample": f"shots at x={','.join(str(sh['sx']) for sh in sample_shots)} of {p.nx}x{p.nz}, "
+                      f"{nsteps} of {p.nt} steps each (forward with strips, backward with imaging), FP32 oracle "
+                      f"on {threads} threads; per-shot times {', '.join(f'{v:.0f}' for v in shot_s)} s "
+                      f"extrapolated linearly in steps to {p.nt}"}
 
 
 if __name__ == "__main__":

@@ -223,6 +223,20 @@ struct Wave {
     }
 };
 
+/* Planted-mistake switch for the CPML pins (tests only, default 0 = none).
+ * Value u in 1..12 swaps the damping profile (ax <-> az) of one memory-variable
+ * update: 1..4 the stress pass's F_xx, F_zz, F_xz, F_zx; 5..12 the velocity
+ * pass's O_C, O_E, O_B, O_A, O_Tx, O_Tz, O_Wz, O_Wx.  A test shows that each
+ * such mistake breaks the absorbing boundary. */
+int g_mutation = 0;
+
+/* Damping profile of memory-variable update u (1-based, as g_mutation). */
+template <class R>
+inline R damp(int u, R ax, R az, bool x_axis) {
+    const bool use_x = (g_mutation == u) ? !x_axis : x_axis;
+    return use_x ? ax : az;
+}
+
 /* Stress pass with CPML, forward_s / rt_s_res (P-S-seperating.cu:316-384,
  * 667-726) on [N, Xn-N) x [N, Zn-N); adds s at (si, sj) after the update. */
 template <class R>
@@ -238,12 +252,14 @@ void stress_cpml(const Setup<R> &S, Wave<R> &W, int si, int sj, R s) {
             R gxz = dpz(W.vx.data(), k, S.c, N, S.dz);
             R gzz = dmz(W.vz.data(), k, S.c, N, S.dz);
             R gzx = dpx(W.vz.data(), k, Zn, S.c, N, S.dx);
-            R ax = S.ax[k], az = S.az[k];
+            const R ax = S.ax[k], az = S.az[k];
+            const R a1 = damp(1, ax, az, true), a2 = damp(2, ax, az, false), a3 = damp(3, ax, az, false),
+                    a4 = damp(4, ax, az, true);
             R &Fxx = W.F[0][k], &Fzz = W.F[1][k], &Fxz = W.F[2][k], &Fzx = W.F[3][k];
-            Fxx = Fxx + (ax * gxx - ax * Fxx) * dt;
-            Fzz = Fzz + (az * gzz - az * Fzz) * dt;
-            Fxz = Fxz + (az * gxz - az * Fxz) * dt;
-            Fzx = Fzx + (ax * gzx - ax * Fzx) * dt;
+            Fxx = Fxx + (a1 * gxx - a1 * Fxx) * dt;
+            Fzz = Fzz + (a2 * gzz - a2 * Fzz) * dt;
+            Fxz = Fxz + (a3 * gxz - a3 * Fxz) * dt;
+            Fzx = Fzx + (a4 * gzx - a4 * Fzx) * dt;
             R vp = S.vp[k], vs = S.vs[k];
             W.th[k] = vp * vp * (gxx + gzz - Fxx - Fzz);
             W.om[k] = vs * vs * (gxz - gzx - Fxz + Fzx);
@@ -275,17 +291,20 @@ void velocity_cpml(const Setup<R> &S, Wave<R> &W) {
             R B = dmx(W.uxz.data(), k, Zn, S.c, N, S.dx);
             R C = dmz(W.uzx.data(), k, S.c, N, S.dz);
             R E = dpx(W.uzz.data(), k, Zn, S.c, N, S.dx);
-            R ax = S.ax[k], az = S.az[k];
+            const R ax = S.ax[k], az = S.az[k];
+            const R b1 = damp(5, ax, az, false), b2 = damp(6, ax, az, true), b3 = damp(7, ax, az, true),
+                    b4 = damp(8, ax, az, false), b5 = damp(9, ax, az, true), b6 = damp(10, ax, az, false),
+                    b7 = damp(11, ax, az, false), b8 = damp(12, ax, az, true);
             R &OC = W.O[0][k], &OE = W.O[1][k], &OB = W.O[2][k], &OA = W.O[3][k];
             R &OTx = W.O[4][k], &OTz = W.O[5][k], &OWz = W.O[6][k], &OWx = W.O[7][k];
-            OC = OC + (az * C - az * OC) * dt;
-            OE = OE + (ax * E - ax * OE) * dt;
-            OB = OB + (ax * B - ax * OB) * dt;
-            OA = OA + (az * A - az * OA) * dt;
-            OTx = OTx + (ax * Tx - ax * OTx) * dt;
-            OTz = OTz + (az * Tz - az * OTz) * dt;
-            OWz = OWz + (az * Wz - az * OWz) * dt;
-            OWx = OWx + (ax * Wx - ax * OWx) * dt;
+            OC = OC + (b1 * C - b1 * OC) * dt;
+            OE = OE + (b2 * E - b2 * OE) * dt;
+            OB = OB + (b3 * B - b3 * OB) * dt;
+            OA = OA + (b4 * A - b4 * OA) * dt;
+            OTx = OTx + (b5 * Tx - b5 * OTx) * dt;
+            OTz = OTz + (b6 * Tz - b6 * OTz) * dt;
+            OWz = OWz + (b7 * Wz - b7 * OWz) * dt;
+            OWx = OWx + (b8 * Wx - b8 * OWx) * dt;
             R dt2 = dt * dt;
             R npx = 2 * W.cur[0][k] - W.prev[0][k] + dt2 * (Tx - OTx + C - E - OC + OE);
             R npz = 2 * W.cur[1][k] - W.prev[1][k] + dt2 * (Tz - OTz + B - A - OB + OA);
@@ -707,7 +726,32 @@ struct Outputs {
      * gradfun accumulates over every backward step into nx*nz arrays */
     const double *observed = nullptr;
     double *grad_vp = nullptr, *grad_vs = nullptr, *illum = nullptr, *misfit = nullptr;
+    /* traces of the backward loop (tests on tiny grids), interior arrays x-major:
+     * fwd_v [nt+1][4][nx][nz] forward source (Vpx, Vpz, Vsx, Vsz) at levels 0..nt;
+     * rcv [nt][4][nx][nz] receiver (Vpx, Vpz, Vsx, Vsz) after the velocity pass
+     * of backward step t; acc [3][nx][nz] num_pp, num_ps, den;
+     * inj [nt][2][Xn] receiver (Vx, Vz) on row Zr right after load_record */
+    double *fwd_v = nullptr, *rcv = nullptr, *acc = nullptr, *inj = nullptr;
+    /* src_v [nt+1][2][nx][nz]: the source P velocity (Vpx, Vpz) level by level as
+     * the backward loop holds it: levels nt and nt-1 handed over from the
+     * forward run, level t <= nt-2 produced by the reconstruction of step t */
+    double *src_v = nullptr;
 };
+
+/* Laplace (P-S-seperating.cu:1457-1469) on interior-only nx x nz arrays
+ * (x-major): the 5-point Laplacian / dh^2 on [2, nx-2) x [2, nz-2), zero on the
+ * 2-cell border (the reference never writes it). */
+template <class R>
+void laplace_interior(int nx, int nz, R dh, const R *img, R *lap) {
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < nz; ++j) {
+            const size_t k = (size_t)i * nz + j;
+            R v = 0;
+            if (i >= 2 && i < nx - 2 && j >= 2 && j < nz - 2)
+                v = (1 / dh / dh) * (img[k + nz] + img[k - nz] - 2 * img[k] + img[k + 1] + img[k - 1] - 2 * img[k]);
+            lap[k] = v;
+        }
+}
 
 /* The whole shot: forward loop (P-S-seperating.cu:2258-2292), mute
  * (P-S-seperating.cu:2293), backward loop (P-S-seperating.cu:2313-2364) and
@@ -757,6 +801,12 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
         if (o.recon_err) { /* px at step t+1 */
             snaps[t + 1] = F.cur[0];
         }
+        if (o.fwd_v)
+            for (int q = 0; q < 4; ++q)
+                for (int ix = 0; ix < nx; ++ix)
+                    for (int iz = 0; iz < nz; ++iz)
+                        o.fwd_v[(((size_t)(t + 1) * 4 + q) * nx + ix) * nz + iz] =
+                            (double)F.cur[q][(size_t)(ix + L) * Zn + iz + L];
         if (rtm) {
             if (t != nt - 1) strips.save(t, F);
             else copy_wave_stress(F, Ssrc); /* read_last_wavefiled2, P-S-seperating.cu:1318-1342 */
@@ -804,6 +854,15 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
     std::vector<R> snew[4], now[4], past[4];
     for (int q = 0; q < 4; ++q) { snew[q] = F.cur[q]; now[q] = F.cur[q]; past[q] = F.prev[q]; }
     std::vector<R> num_pp(n, (R)0), num_ps(n, (R)0), den(n, (R)0);
+    auto trace_src = [&](int lev, const std::vector<R> *v) {
+        if (!o.src_v) return;
+        for (int q = 0; q < 2; ++q)
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iz = 0; iz < nz; ++iz)
+                    o.src_v[(((size_t)lev * 2 + q) * nx + ix) * nz + iz] = (double)v[q][(size_t)(ix + L) * Zn + iz + L];
+    };
+    trace_src(nt, now);
+    if (nt >= 1) trace_src(nt - 1, past);
     /* FWI: source totals at level t+1 for backward step t (V^nt first), and
      * the gradient accumulators */
     std::vector<R> u1[2], gvp, gvs, ill;
@@ -822,6 +881,7 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
     for (int t = nt - 1; t >= 0; --t) {
         if (t < nt - 1) {
             recon_velocity(S, Ssrc, snew, now, past);
+            trace_src(t, past);
             if (!exact) {
                 std::vector<R> *dv[2] = {&Ssrc.vx, &Ssrc.vz};
                 strips.restore(t, RS_VX, RS_VZ + 1, dv);
@@ -852,6 +912,11 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
             Rw.vx[k] = rec[0][(size_t)ix * nt + t];
             Rw.vz[k] = rec[1][(size_t)ix * nt + t];
         }
+        if (o.inj)
+            for (int i = 0; i < g.Xn; ++i) {
+                o.inj[((size_t)t * 2 + 0) * g.Xn + i] = (double)Rw.vx[(size_t)i * Zn + Zr];
+                o.inj[((size_t)t * 2 + 1) * g.Xn + i] = (double)Rw.vz[(size_t)i * Zn + Zr];
+            }
         stress_cpml(S, Rw, -1, -1, (R)0);
         if (fwi) {
             gradfun_step(S, u1[0].data(), u1[1].data(), Rw.th.data(), Rw.om.data(), Rw.uxx.data(), Rw.uxz.data(),
@@ -878,7 +943,20 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
                 den[k] += spx * spx + spz * spz;
                 num_ps[k] += spx * Rw.cur[2][k] + spz * Rw.cur[3][k];
             }
+        if (o.rcv)
+            for (int q = 0; q < 4; ++q)
+                for (int ix = 0; ix < nx; ++ix)
+                    for (int iz = 0; iz < nz; ++iz)
+                        o.rcv[(((size_t)t * 4 + q) * nx + ix) * nz + iz] = (double)Rw.cur[q][(size_t)(ix + L) * Zn + iz + L];
     }
+    if (o.acc)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                const size_t k = (size_t)(ix + L) * Zn + iz + L, oi = (size_t)ix * nz + iz;
+                o.acc[oi] = (double)num_pp[k];
+                o.acc[(size_t)nx * nz + oi] = (double)num_ps[k];
+                o.acc[(size_t)2 * nx * nz + oi] = (double)den[k];
+            }
     if (fwi)
         for (int i = L; i < g.Xn - L; ++i)
             for (int j = L; j < Zn - L; ++j) {
@@ -896,20 +974,21 @@ int run_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, const d
             ips[k] = num_ps[k] / den[k];
         }
     const R dh = S.dh;
-    for (int i = L; i < g.Xn - L; ++i)
-        for (int j = L; j < Zn - L; ++j) {
-            size_t k = (size_t)i * Zn + j;
-            size_t oi = (size_t)(i - L) * nz + (j - L);
-            if (o.img_pp) o.img_pp[oi] = (double)ipp[k];
-            if (o.img_ps) o.img_ps[oi] = (double)ips[k];
-            R lpp = 0, lps = 0;
-            if (i >= L + 2 && i < g.Xn - L - 2 && j >= L + 2 && j < Zn - L - 2) {
-                lpp = (1 / dh / dh) * (ipp[k + Zn] + ipp[k - Zn] - 2 * ipp[k] + ipp[k + 1] + ipp[k - 1] - 2 * ipp[k]);
-                lps = (1 / dh / dh) * (ips[k + Zn] + ips[k - Zn] - 2 * ips[k] + ips[k + 1] + ips[k - 1] - 2 * ips[k]);
-            }
-            if (o.lap_pp) o.lap_pp[oi] = (double)lpp;
-            if (o.lap_ps) o.lap_ps[oi] = (double)lps;
+    std::vector<R> cpp((size_t)nx * nz), cps((size_t)nx * nz), lpp((size_t)nx * nz), lps((size_t)nx * nz);
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) {
+            const size_t k = (size_t)(ix + L) * Zn + iz + L, oi = (size_t)ix * nz + iz;
+            cpp[oi] = ipp[k];
+            cps[oi] = ips[k];
         }
+    laplace_interior(nx, nz, dh, cpp.data(), lpp.data());
+    laplace_interior(nx, nz, dh, cps.data(), lps.data());
+    for (size_t oi = 0; oi < (size_t)nx * nz; ++oi) {
+        if (o.img_pp) o.img_pp[oi] = (double)cpp[oi];
+        if (o.img_ps) o.img_ps[oi] = (double)cps[oi];
+        if (o.lap_pp) o.lap_pp[oi] = (double)lpp[oi];
+        if (o.lap_ps) o.lap_ps[oi] = (double)lps[oi];
+    }
     return OUC_OK;
 }
 
@@ -1035,6 +1114,43 @@ int oracle_rtm_shot(const ouc_params *p, const ouc_model *m, const ouc_shot *s, 
     Outputs o;
     o.img_pp = img_pp; o.img_ps = img_ps; o.lap_pp = lap_pp; o.lap_ps = lap_ps; o.rec_out = rec_out;
     return precision == 32 ? run_shot<float>(p, m, s, rec_in, true, o) : run_shot<double>(p, m, s, rec_in, true, o);
+}
+
+/* RTM of one shot with the backward-loop traces (see Outputs); any trace
+ * pointer may be null.  Meant for tiny grids. */
+int oracle_rtm_trace(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision, const double *rec_in,
+                     double *img_pp, double *img_ps, double *fwd_v, double *src_v, double *rcv, double *acc,
+                     double *inj) {
+    if (!p || !m || !s) return OUC_INVALID_ARG;
+    Outputs o;
+    o.img_pp = img_pp; o.img_ps = img_ps;
+    o.fwd_v = fwd_v; o.src_v = src_v; o.rcv = rcv; o.acc = acc; o.inj = inj;
+    if (fwd_v) std::fill(fwd_v, fwd_v + (size_t)4 * p->nx * p->nz, 0.0); /* level 0 */
+    return precision == 32 ? run_shot<float>(p, m, s, rec_in, true, o) : run_shot<double>(p, m, s, rec_in, true, o);
+}
+
+/* Forward modelling only, with the interior split velocities of every level:
+ * fwd_v [nt+1][4][nx][nz] (Vpx, Vpz, Vsx, Vsz; level 0 is zero). */
+int oracle_forward_trace(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision, double *fwd_v) {
+    if (!p || !m || !s || !fwd_v) return OUC_INVALID_ARG;
+    Outputs o;
+    o.fwd_v = fwd_v;
+    std::fill(fwd_v, fwd_v + (size_t)4 * p->nx * p->nz, 0.0);
+    return precision == 32 ? run_shot<float>(p, m, s, nullptr, false, o) : run_shot<double>(p, m, s, nullptr, false, o);
+}
+
+/* Laplace (P-S-seperating.cu:1457-1469) of an interior image, FP64. */
+int oracle_laplace(int nx, int nz, double dh, const double *img, double *lap) {
+    if (nx <= 0 || nz <= 0 || !img || !lap || !(dh > 0)) return OUC_INVALID_ARG;
+    laplace_interior(nx, nz, dh, img, lap);
+    return OUC_OK;
+}
+
+/* Planted-mistake switch of the CPML memory-variable updates (tests only). */
+int oracle_set_mutation(int u) {
+    if (u < 0 || u > 12) return OUC_INVALID_ARG;
+    g_mutation = u;
+    return OUC_OK;
 }
 
 /* Split-PML first-order engine, forward seismograms [6][nx][nt]. */

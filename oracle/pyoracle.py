@@ -46,6 +46,11 @@ def lib():
         L.oracle_fwi_gradient.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp]
         L.oracle_gradfun.argtypes = [P(abi.Params), P(abi.Model)] + [dp] * 11
         L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_rtm_trace.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp, dp, dp,
+                                       dp]
+        L.oracle_forward_trace.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
+        L.oracle_laplace.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp]
+        L.oracle_set_mutation.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -212,3 +217,61 @@ def recon_error(p, vp, vs, shot, precision=64):
     err = np.zeros(p.nt, np.float64)
     _check(lib().oracle_recon_error(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(err)), "recon")
     return err
+
+
+def rtm_trace(p, vp, vs, shot, precision=64, rec_in=None):
+    """RTM with the backward-loop traces (tiny grids only): images, forward
+    source split velocities fwd_v [nt+1][4][nx][nz] (px, pz, sx, sz), the
+    source P velocity as the backward loop holds it src_v [nt+1][2][nx][nz]
+    (levels nt, nt-1 handed over, level t reconstructed at step t), receiver split parts after each
+    backward step's velocity pass rcv [nt][4][nx][nz] (pxr, pzr, sxr, szr),
+    accumulators acc [3][nx][nz] (num_pp, num_ps, den) and the receiver row
+    right after injection inj [nt][2][Xn]."""
+    m = _ModelRef(vp, vs)
+    s = _ShotRef(**shot)
+    nx, nz, nt = p.nx, p.nz, p.nt
+    Xn = nx + 2 * p.npml
+    out = {"img_pp": np.zeros((nx, nz)), "img_ps": np.zeros((nx, nz)),
+           "fwd_v": np.zeros((nt + 1, 4, nx, nz)), "src_v": np.zeros((nt + 1, 2, nx, nz)), "rcv": np.zeros((nt, 4, nx, nz)),
+           "acc": np.zeros((3, nx, nz)), "inj": np.zeros((nt, 2, Xn))}
+    rin = None if rec_in is None else np.ascontiguousarray(rec_in, np.float64)
+    _check(lib().oracle_rtm_trace(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(rin),
+                                  *[abi.dptr(out[k]) for k in ("img_pp", "img_ps", "fwd_v", "src_v", "rcv", "acc", "inj")]),
+           "rtm trace")
+    return out
+
+
+def forward_trace(p, vp, vs, shot, precision=64):
+    """Interior split velocities of every forward level: [nt+1][4][nx][nz]
+    (px, pz, sx, sz; level 0 is zero).  Tiny grids only."""
+    m = _ModelRef(vp, vs)
+    s = _ShotRef(**shot)
+    out = np.zeros((p.nt + 1, 4, p.nx, p.nz))
+    _check(lib().oracle_forward_trace(C.byref(p), C.byref(m.m), C.byref(s.s), precision, abi.dptr(out)),
+           "forward trace")
+    return out
+
+
+def laplace(img, dh):
+    """The reference's Laplace filter (5-point / dh^2, 2-cell zero border) of
+    an interior image, FP64."""
+    img = np.ascontiguousarray(img, np.float64)
+    out = np.zeros_like(img)
+    _check(lib().oracle_laplace(img.shape[0], img.shape[1], float(dh), abi.dptr(img), abi.dptr(out)), "laplace")
+    return out
+
+
+class mutation:
+    """Context manager planting mistake u (1..12: ax/az swapped in one CPML
+    memory-variable update) in the oracle, for pins that must catch it."""
+
+    def __init__(self, u):
+        self.u = int(u)
+
+    def __enter__(self):
+        _check(lib().oracle_set_mutation(self.u), "set_mutation")
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_mutation(0)
+        return False

@@ -63,3 +63,42 @@ def measured_traffic(kernel_name, nshots=1):
                 except ValueError:
                     return None
     return None
+
+
+def fused_regions(p):
+    """Cell counts per shot of the fused engine's kernel regions (CPML
+    contexts): the update region, its core, the CPML band around the core, the
+    interior, the reconstruction core [L+2N, Xn-L-2N)^2 and band, and the
+    strip ring (four strips 2N cells wide straddling the interior's edge)."""
+    N = p.half_order
+    Xn, Zn, upd, interior = _cells(p)
+    core = max(0, p.nx - 2 * N) * max(0, p.nz - 2 * N)
+    rcore = max(0, p.nx - 4 * N) * max(0, p.nz - 4 * N)
+    ring = 2 * (2 * N * (p.nx + 2 * N)) + 2 * (2 * N * max(0, p.nz - 2 * N))
+    return {"update": upd, "core": core, "band": upd - core, "interior": interior, "recon_core": rcore,
+            "recon_band": interior - rcore, "ring": ring}
+
+
+# Algorithmic bytes per cell of each fused kernel (FP32 operands once):
+#   k_core step: V, DV, S, DS (x, z) read and written, vp^2, vs^2 read  = 72
+#   k_core imaging: + source P level (2) read, 3 accumulators RMW     = 104
+#   k_edge (CPML band): the core's 72 + ax, az + 4 F RMW + 8 O RMW    = 176
+#   reconstruction (core mode and band kernels): V, DV, S, DS RMW,
+#     vp^2, vs^2, + source P level (2) written                        = 80
+#   strip save, per ring cell and field: the value read, FP16 residual
+#     written, encoder state (prediction, fed-back error) RMW        = 22 (FP32 strips: 8)
+#   strip decode, per ring cell and field: residual read, previous
+#     decoded value read, decoded value written                       = 10 (FP32 strips: 0)
+CORE_STEP_B, CORE_IMAGE_B, EDGE_B, RECON_B = 72.0, 104.0, 176.0, 80.0
+
+
+def fused_loop_bytes(p, nshots, fp16):
+    """Algorithmic bytes of one batch's forward loop and backward loop (all
+    steps, all kernels), for the loop-level rates next to the roofline."""
+    r = fused_regions(p)
+    nf = 8
+    save = r["ring"] * nf * (22.0 if fp16 else 8.0)
+    dec = r["ring"] * nf * (10.0 if fp16 else 0.0)
+    fwd = r["core"] * CORE_STEP_B + r["band"] * EDGE_B + save
+    bwd = (r["core"] * CORE_IMAGE_B + r["band"] * EDGE_B + r["interior"] * RECON_B + dec)
+    return nshots * p.nt * fwd, nshots * (p.nt - 1) * bwd

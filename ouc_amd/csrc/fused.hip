@@ -97,15 +97,21 @@ struct FusedState {
     // records [cap][2][nt][nx], residual partial sums, shot-summed interior outputs
     DevBuf gacc, obs, part, gout;
     // time loops captured as HIP graphs, replayed while the key matches
+    // The key holds every host value baked into the captured kernel arguments
+    // that can change between calls on one context: the batch size, the loop
+    // kind, the recorded components and the receiver row (per-shot positions,
+    // wavelets and start steps travel through device memory).
     struct LoopGraph {
-        int B = -1, rtm = -1, ncomp = -1;
+        int B = -1, rtm = -1, ncomp = -1, zr = -1;
         hipGraphExec_t exec = nullptr;
         void drop() {
             if (exec) (void)hipGraphExecDestroy(exec);
             exec = nullptr;
             B = -1;
         }
-        bool matches(int b, int r, int n) const { return exec && B == b && rtm == r && ncomp == n; }
+        bool matches(int b, int r, int n, int z) const {
+            return exec && B == b && rtm == r && ncomp == n && zr == z;
+        }
     } gfwd, gbwd;
     void drop_graphs() { gfwd.drop(); gbwd.drop(); }
     // side stream (default priority) for the reconstruction, which runs
@@ -129,12 +135,12 @@ struct FusedState {
 // Enqueues `body` on stream s directly, or replays the graph it was captured
 // into the first time for this key (HIP stream capture).
 template <class F>
-void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int rtm, int ncomp, F &&body) {
+void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int rtm, int ncomp, int zr, F &&body) {
     if (!graph) {
         body();
         return;
     }
-    if (!lg.matches(B, rtm, ncomp)) {
+    if (!lg.matches(B, rtm, ncomp, zr)) {
         lg.drop();
         hipGraph_t gr = nullptr;
         OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -151,7 +157,7 @@ void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int r
         (void)hipGraphDestroy(gr);
         OUC_HIP(e);
         lg.exec = ex;
-        lg.B = B; lg.rtm = rtm; lg.ncomp = ncomp;
+        lg.B = B; lg.rtm = rtm; lg.ncomp = ncomp; lg.zr = zr;
     }
     OUC_HIP(hipGraphLaunch(lg.exec, s));
 }
@@ -586,7 +592,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     const bool graph_b = c->graphs && !snap && !rsnap;
     if (graph_b) st->ensure_side(); // the captured backward loop runs independent work on two streams
     int rb = 0, ksnap = 0;
-    run_loop(s, graph, st->gfwd, B, rtm, ncomp, [&] {
+    run_loop(s, graph, st->gfwd, B, rtm, ncomp, zr, [&] {
     for (int t = 0; t < nt; ++t) {
         a.t = t;
         a.rb = rb;
@@ -664,7 +670,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     // stream: iteration t decodes slot t, and the reconstruction of step t reads
     // slots t+1 and (EXACT) t
     const dim3 dgrid(rblocks, RS_NF, B);
-    run_loop(s, graph_b, st->gbwd, B, fwi ? 2 : 1, ncomp, [&] {
+    run_loop(s, graph_b, st->gbwd, B, fwi ? 2 : 1, ncomp, zr, [&] {
     for (int t = nt - 1; t >= 0; --t) {
         if (graph_b) {
             OUC_HIP(hipEventRecord(st->evf, s));
