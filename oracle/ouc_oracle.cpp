@@ -324,25 +324,40 @@ void velocity_cpml(const Setup<R> &S, Wave<R> &W) {
  * and restores them in the backward loop (P-S-seperating.cu:1344-1424).  Only
  * cells of the frame [L-N, Xn-L+N)^2 that lie within N of the interior edge are
  * ever read by the interior-only reconstruction, so the ring stores exactly
- * those: four rectangles, top/bottom (nx+2N) x 2N, left/right 2N x (nz-2N). */
+ * those: four rectangles, top/bottom (nx+2N) x 2N, left/right 2N x (nz-2N).
+ * Storage order: each rectangle's outer part (its N-1 outermost pad rows or
+ * columns) for top, bottom, left, right, then their inner parts (the N
+ * interior rows or columns next to the edge and the first pad row or column);
+ * z runs fastest within a part. */
 struct Ring {
     int L, N, Xn, Zn, nx, nz;
-    size_t ntop, nside, ncells;
+    size_t ntop, nside, ncells, nouter;
     void init(const Geo &g) {
         L = g.L; N = g.N; Xn = g.Xn; Zn = g.Zn; nx = g.nx; nz = g.nz;
         ntop = (size_t)(nx + 2 * N) * (2 * N);
         nside = (size_t)(2 * N) * (nz - 2 * N);
         ncells = 2 * ntop + 2 * nside;
+        nouter = (size_t)(N - 1) * (2 * (nx + 2 * N) + 2 * (nz - 2 * N));
     }
+    /* Part (outer, inner) of one side: the first local row (top, bottom) or
+     * column (left, right) across the strip's 2N, and the part's width. */
     void cell(size_t r, int &i, int &j) const {
-        if (r < ntop) { i = L - N + (int)(r / (2 * N)); j = L - N + (int)(r % (2 * N)); return; }
-        r -= ntop;
-        if (r < ntop) { i = L - N + (int)(r / (2 * N)); j = Zn - L - N + (int)(r % (2 * N)); return; }
-        r -= ntop;
-        int w = nz - 2 * N;
-        if (r < nside) { i = L - N + (int)(r / w); j = L + N + (int)(r % w); return; }
-        r -= nside;
-        i = Xn - L - N + (int)(r / w); j = L + N + (int)(r % w);
+        const size_t cw = (size_t)(nx + 2 * N), w = (size_t)(nz - 2 * N);
+        const bool inner = r >= nouter;
+        if (inner) r -= nouter;
+        const size_t wd = inner ? (size_t)(N + 1) : (size_t)(N - 1);
+        /* local offsets of the part within the 2N-wide strip: top/left keep
+         * the pad side first (outer rows 0..N-2, inner N-1..2N-1), bottom/right
+         * the interior side first (inner 0..N, outer N+1..2N-1) */
+        const int lo_tl = inner ? N - 1 : 0, lo_br = inner ? 0 : N + 1;
+        if (r < cw * wd) { i = L - N + (int)(r / wd); j = L - N + lo_tl + (int)(r % wd); return; }
+        r -= cw * wd;
+        if (r < cw * wd) { i = L - N + (int)(r / wd); j = Zn - L - N + lo_br + (int)(r % wd); return; }
+        r -= cw * wd;
+        if (r < wd * w) { i = L - N + lo_tl + (int)(r / w); j = L + N + (int)(r % w); return; }
+        r -= wd * w;
+        i = Xn - L - N + lo_br + (int)(r / w);
+        j = L + N + (int)(r % w);
     }
 };
 
@@ -503,21 +518,47 @@ struct Codec16 {
     }
 };
 
+/* Fields whose inner ring part the FP16 store keeps exactly (in FP32):
+ * theta and the four gradients.  The reconstruction integrates the split
+ * source velocities of the ring's interior rows open loop from them, so their
+ * rounding accumulates in the imaged source field; omega (S part only) and the
+ * velocities (closed-loop stress) stay residual-coded on the whole ring. */
+inline bool inner32_field(int f) {
+    return f == RS_TH || f == RS_UZX || f == RS_UZZ || f == RS_UXX || f == RS_UXZ;
+}
+inline int inner32_rank(int f) {
+    int k = 0;
+    for (int a = 0; a < f; ++a) k += inner32_field(a) ? 1 : 0;
+    return k;
+}
+constexpr int kNInner32 = 5;
+
 template <class R>
 struct Strips {
     Ring ring;
     int nslots = 0;
     bool fp16 = false;
-    std::vector<R> data; /* FP32 mode: [slot][field][ring] */
-    Codec16 codec;       /* FP16 mode; slots are saved in time order, restored in reverse */
+    std::vector<R> data;          /* FP32 mode: [slot][field][ring] */
+    std::vector<Codec16> codec;   /* FP16 mode, one per field (slots saved in time order, restored in reverse):
+                                     the whole ring, or its outer part for the FP32-kept fields */
+    std::vector<float> in32;      /* FP16 mode: [slot][kNInner32][inner cells] */
+    size_t ninner = 0;
     void init(const Geo &g, int slots, bool use_fp16) {
         ring.init(g);
         nslots = slots;
         fp16 = use_fp16;
-        if (fp16) codec.init(ring.ncells, slots, RS_NF, std::vector<int>(RS_NF, slots - 1));
-        else data.assign((size_t)slots * RS_NF * ring.ncells, (R)0);
+        ninner = ring.ncells - ring.nouter;
+        if (fp16) {
+            codec.resize(RS_NF);
+            for (int f = 0; f < RS_NF; ++f)
+                codec[f].init(inner32_field(f) ? ring.nouter : ring.ncells, slots, 1, std::vector<int>(1, slots - 1));
+            in32.assign((size_t)slots * kNInner32 * ninner, 0.f);
+        } else {
+            data.assign((size_t)slots * RS_NF * ring.ncells, (R)0);
+        }
     }
     R *slot(int t, int f) { return data.data() + ((size_t)t * RS_NF + f) * ring.ncells; }
+    float *inner(int t, int f) { return in32.data() + ((size_t)t * kNInner32 + inner32_rank(f)) * ninner; }
     void save(int t, const Wave<R> &W) {
         const std::vector<R> *src[RS_NF] = {&W.vx, &W.vz, &W.th, &W.om, &W.uzx, &W.uzz, &W.uxx, &W.uxz};
         std::vector<float> v(fp16 ? ring.ncells : 0);
@@ -529,18 +570,26 @@ struct Strips {
                 if (fp16) v[r] = (float)x;
                 else slot(t, f)[r] = x;
             }
-            if (fp16) codec.encode(t, f, v.data());
+            if (fp16) {
+                codec[f].encode(t, 0, v.data());
+                if (inner32_field(f)) std::copy(v.begin() + ring.nouter, v.end(), inner(t, f));
+            }
         }
     }
     void restore(int t, int f0, int f1, std::vector<R> *const *dst) {
         for (int f = f0; f < f1; ++f) {
             std::vector<R> &out = *dst[f - f0];
-            const float *d16 = fp16 ? codec.decoded(t, f) : nullptr;
+            const float *d16 = fp16 ? codec[f].decoded(t, 0) : nullptr;
+            const float *in = fp16 && inner32_field(f) ? inner(t, f) : nullptr;
             const R *d = fp16 ? nullptr : slot(t, f);
             for (size_t r = 0; r < ring.ncells; ++r) {
                 int i, j;
                 ring.cell(r, i, j);
-                out[(size_t)i * ring.Zn + j] = fp16 ? (R)d16[r] : d[r];
+                R v;
+                if (!fp16) v = d[r];
+                else if (in && r >= ring.nouter) v = (R)in[r - ring.nouter];
+                else v = (R)d16[r];
+                out[(size_t)i * ring.Zn + j] = v;
             }
         }
     }

@@ -75,7 +75,7 @@ ouc::KModel ouc_ctx::kmodel() const { return KModel{p2.f(), q2.f(), ax.f(), az.f
 ouc::KRing ouc_ctx::kring() const {
     KRing r;
     r.L = ring.L; r.N = ring.N; r.Xn = ring.Xn; r.Zn = ring.Zn; r.nx = ring.nx; r.nz = ring.nz;
-    r.ntop = ring.ntop; r.nside = ring.nside; r.ncells = ring.ncells;
+    r.ntop = ring.ntop; r.nside = ring.nside; r.ncells = ring.ncells; r.nouter = ring.nouter;
     return r;
 }
 
@@ -83,11 +83,13 @@ ouc::KStrips ouc_ctx::kstrips() const {
     KStrips s{};
     s.f32 = strips32.f();
     s.f16 = strips16.as<__half>();
+    s.in32 = strips_in32.f();
     s.e8 = stripse8.as<int8_t>();
     s.key = stripskey.f();
     s.err = stripserr.f();
     s.dec = stripsdec.f();
     s.ncells = ring.ncells;
+    s.nouter = ring.nouter;
     s.nchunks = strip_chunks;
     s.fp16 = p.strip_precision == OUC_STRIP_FP16;
     s.last_v = s.last_th = (int)strip_slots - 1; // slots 0..nt-2 of both field groups
@@ -99,10 +101,11 @@ void ouc_ctx::ensure_strips() {
     int64_t slots = g.nt - 1;
     if (strip_fp16 == want16 && strip_slots == slots) return;
     strips32.release(); strips16.release(); stripse8.release(); stripskey.release(); stripserr.release();
-    stripsdec.release();
+    stripsdec.release(); strips_in32.release();
     strip_chunks = (ring.ncells + 63) / 64;
     if (want16) {
-        strips16.alloc((size_t)slots * RS_NF * ring.ncells * sizeof(__half));
+        strips16.alloc((size_t)slots * strip16_offset(ring.ncells, ring.nouter, RS_NF) * sizeof(__half));
+        strips_in32.alloc((size_t)slots * kNInner32 * (ring.ncells - ring.nouter) * sizeof(float));
         stripse8.alloc((size_t)slots * RS_NF * strip_chunks);
         stripskey.alloc((size_t)((slots + kStripKey - 1) / kStripKey) * RS_NF * ring.ncells * sizeof(float));
         stripserr.alloc((size_t)2 * RS_NF * ring.ncells * sizeof(float));
@@ -600,6 +603,7 @@ ouc_status ouc_strip_codec_fp16(int32_t device, const float *x, int64_t n, int32
         s.err = derr.f();
         s.dec = ddec.f();
         s.ncells = n;
+        s.nouter = n; // every cell residual-coded
         s.nchunks = chunks;
         s.fp16 = 1;
         s.last_v = s.last_th = steps - 1;

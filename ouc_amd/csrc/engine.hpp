@@ -78,6 +78,7 @@ struct ouc_ctx {
     ouc::DevBuf counts; // nx
     ouc::DevBuf img[4]; // pp, ps, pp_lap, ps_lap (nx*nz)
     ouc::DevBuf strips32, strips16, stripse8; // FP32 strips, or FP16 residuals and chunk exponents
+    ouc::DevBuf strips_in32;                     // FP16: inner ring part of the FP32-kept fields
     ouc::DevBuf stripskey, stripserr, stripsdec; // FP16: keyframes, encoder state [2][RS_NF][ring], decoded [2][RS_NF][ring]
     int64_t strip_slots = 0, strip_chunks = 0;
     int strip_fp16 = -1;

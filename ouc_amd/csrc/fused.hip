@@ -76,7 +76,8 @@ struct FusedState {
     DevBuf acc[3];         // npp, nps, den
     DevBuf strips32, strips16, e8;   // FP32 strips, or FP16 residuals and chunk exponents
     DevBuf keyb, ef, decb;           // FP16: keyframes, encoder state, decoded slots ([cap][2][RS_NF][ring])
-    long shot_strip = 0, shot_chunks = 0, shot_key = 0, shot_aux = 0, nchunks = 0;
+    DevBuf in32;                     // FP16: inner ring part of the FP32-kept fields
+    long shot_strip = 0, shot_chunks = 0, shot_key = 0, shot_aux = 0, shot_in32 = 0, nchunks = 0;
     int nslots = 0, fp16 = 0;
     DevBuf rec;            // [cap][2][nt][nx]
     DevBuf srcpos, wav, counts, t0;
@@ -266,8 +267,12 @@ size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
     const Grid &g = c->g;
     const size_t plane = g.pitched() * sizeof(float);
     size_t b = 2 * kWaveArrays * plane + 6 * plane + 3 * plane;
-    const size_t cells = (size_t)c->ring.ncells;
-    b += (size_t)g.nt * RS_NF * cells * (fp16 ? sizeof(__half) : sizeof(float));
+    const size_t cells = (size_t)c->ring.ncells, outer = (size_t)c->ring.nouter;
+    if (fp16)
+        b += (size_t)g.nt * (strip16_offset(cells, outer, RS_NF) * sizeof(__half) +
+                             kNInner32 * (cells - outer) * sizeof(float));
+    else
+        b += (size_t)g.nt * RS_NF * cells * sizeof(float);
     if (fp16)
         b += (size_t)g.nt * RS_NF * ((cells + 63) / 64) +
              (size_t)((g.nt + kStripKey - 1) / kStripKey) * RS_NF * cells * sizeof(float) +
@@ -296,14 +301,17 @@ void ensure(ouc_ctx *c, int shots) {
     for (auto &b : st->acc) b.alloc(span);
     st->nslots = g.nt;
     st->nchunks = (c->ring.ncells + 63) / 64;
-    st->shot_strip = (long)st->nslots * RS_NF * c->ring.ncells;
+    st->shot_strip = (long)st->nslots * (want16 ? strip16_offset(c->ring.ncells, c->ring.nouter, RS_NF)
+                                                : (long)RS_NF * c->ring.ncells);
+    st->shot_in32 = want16 ? (long)st->nslots * kNInner32 * (c->ring.ncells - c->ring.nouter) : 0;
     st->shot_chunks = (long)st->nslots * RS_NF * st->nchunks;
     st->shot_key = (long)((st->nslots + kStripKey - 1) / kStripKey) * RS_NF * c->ring.ncells;
     st->shot_aux = (long)2 * RS_NF * c->ring.ncells;
     st->strips32.release(); st->strips16.release(); st->e8.release();
-    st->keyb.release(); st->ef.release(); st->decb.release();
+    st->keyb.release(); st->ef.release(); st->decb.release(); st->in32.release();
     if (want16) {
         st->strips16.alloc((size_t)st->shot_strip * shots * sizeof(__half));
+        st->in32.alloc((size_t)st->shot_in32 * shots * sizeof(float));
         st->e8.alloc((size_t)st->shot_chunks * shots);
         st->keyb.alloc((size_t)st->shot_key * shots * sizeof(float));
         st->ef.alloc((size_t)st->shot_aux * shots * sizeof(float));
@@ -387,10 +395,12 @@ KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
     KStrips s{};
     s.f32 = st->strips32.f();
     s.f16 = st->strips16.as<__half>();
+    s.in32 = st->in32.f();
     s.e8 = st->e8.as<int8_t>();
     s.key = st->keyb.f();
     s.err = st->ef.f();
     s.dec = st->decb.f();
+    s.nouter = c->ring.nouter;
     s.last_v = st->nslots - 2;  // velocity totals V^{t+1} are saved for t < nt-1
     s.last_th = st->nslots - 1; // the stress-like fields for every t
     s.ncells = c->ring.ncells;
@@ -575,7 +585,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     unsigned int *mon_f = st->mon.as<unsigned int>(), *mon_r = mon_f + 2 * st->cap;
     const KRing kr = c->kring();
     const KStrips ks0 = base_strips(st, c);
-    const KStripStride sstride{st->shot_strip, st->shot_chunks, st->shot_key, st->shot_aux};
+    const KStripStride sstride{st->shot_strip, st->shot_chunks, st->shot_key, st->shot_aux, st->shot_in32};
     const int rblocks = (int)((c->ring.ncells + 255) / 256);
     int nlaunch = 0;
 
@@ -768,7 +778,7 @@ int default_capacity(ouc_ctx *c, int nshots, size_t extra_per_shot = 0) {
     OUC_HIP(hipMemGetInfo(&freeb, &total));
     // keep what this context already holds available for reuse
     FusedState *st = fs(c);
-    size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->e8.bytes +
+    size_t held = st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->in32.bytes + st->e8.bytes +
                   st->keyb.bytes + st->ef.bytes + st->decb.bytes + st->gacc.bytes + st->obs.bytes;
     size_t avail = (size_t)((freeb + held) * 0.85);
     size_t per = per_shot_bytes(c, c->p.strip_precision == OUC_STRIP_FP16) + extra_per_shot;

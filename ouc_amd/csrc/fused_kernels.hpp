@@ -161,17 +161,6 @@ struct FRecon {
     float *vpx, *vpz;    // [shot][plane] output level, or NULL (no imaging condition)
 };
 
-__device__ __forceinline__ long ring_index(const KRing &r, int i, int j) {
-    const int L = r.L, N = r.N;
-    if (i < L - N || i >= r.Xn - L + N || j < L - N || j >= r.Zn - L + N) return -1;
-    if (i >= L + N && i < r.Xn - L - N && j >= L + N && j < r.Zn - L - N) return -1;
-    if (j < L + N) return (long)(i - (L - N)) * (2 * N) + (j - (L - N));
-    if (j >= r.Zn - L - N) return r.ntop + (long)(i - (L - N)) * (2 * N) + (j - (r.Zn - L - N));
-    const int w = r.nz - 2 * N;
-    if (i < L + N) return 2 * r.ntop + (long)(i - (L - N)) * w + (j - (L + N));
-    return 2 * r.ntop + r.nside + (long)(i - (r.Xn - L - N)) * w + (j - (L + N));
-}
-
 __device__ __forceinline__ KStrips shot_strips(const FRecon &rc) { return strips_of_shot(rc.strips, rc.stride, blockIdx.z); }
 
 // Totals seen by the reconstruction's stress pass at (i, j).

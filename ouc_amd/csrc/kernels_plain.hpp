@@ -279,22 +279,70 @@ static __global__ void k_mute(int nx, int nt, int ncomp, const int32_t *__restri
 }
 
 // ---- strips ----------------------------------------------------------------
+// The ring: four strips 2N cells wide straddling the interior's edge (top and
+// bottom (nx+2N) x 2N, left and right 2N x (nz-2N)).  Each strip splits into
+// an outer part, its N-1 outermost pad rows (columns), and an inner part, the
+// N interior rows next to the edge plus the first pad row.  Storage order: the
+// outer parts of top, bottom, left, right, then their inner parts; within a
+// part z runs fastest (one column of a top/bottom part, one z line of a
+// left/right part at a time).
 struct KRing {
     int L, N, Xn, Zn, nx, nz;
-    long ntop, nside, ncells;
+    long ntop, nside, ncells, nouter;
 };
 
-__device__ __forceinline__ void ring_cell(const KRing &r, long q, int &i, int &j) {
-    const int w2 = 2 * r.N;
-    if (q < r.ntop) { i = r.L - r.N + (int)(q / w2); j = r.L - r.N + (int)(q % w2); return; }
-    q -= r.ntop;
-    if (q < r.ntop) { i = r.L - r.N + (int)(q / w2); j = r.Zn - r.L - r.N + (int)(q % w2); return; }
-    q -= r.ntop;
-    const int w = r.nz - 2 * r.N;
-    if (q < r.nside) { i = r.L - r.N + (int)(q / w); j = r.L + r.N + (int)(q % w); return; }
-    q -= r.nside;
-    i = r.Xn - r.L - r.N + (int)(q / w);
-    j = r.L + r.N + (int)(q % w);
+__host__ __device__ __forceinline__ void ring_cell(const KRing &r, long q, int &i, int &j) {
+    const int N = r.N, L = r.L, w = r.nz - 2 * N;
+    const long cw = r.nx + 2 * N;
+    if (q < r.nouter) {
+        const int wo = N - 1;
+        if (q < cw * wo) { i = L - N + (int)(q / wo); j = L - N + (int)(q % wo); return; }
+        q -= cw * wo;
+        if (q < cw * wo) { i = L - N + (int)(q / wo); j = r.Zn - L + 1 + (int)(q % wo); return; }
+        q -= cw * wo;
+        if (q < (long)wo * w) { i = L - N + (int)(q / w); j = L + N + (int)(q % w); return; }
+        q -= (long)wo * w;
+        i = r.Xn - L + 1 + (int)(q / w);
+        j = L + N + (int)(q % w);
+        return;
+    }
+    q -= r.nouter;
+    const int wi = N + 1;
+    if (q < cw * wi) { i = L - N + (int)(q / wi); j = L - 1 + (int)(q % wi); return; }
+    q -= cw * wi;
+    if (q < cw * wi) { i = L - N + (int)(q / wi); j = r.Zn - L - N + (int)(q % wi); return; }
+    q -= cw * wi;
+    if (q < (long)wi * w) { i = L - 1 + (int)(q / w); j = L + N + (int)(q % w); return; }
+    q -= (long)wi * w;
+    i = r.Xn - L - N + (int)(q / w);
+    j = L + N + (int)(q % w);
+}
+
+// Storage index of padded cell (i, j) in the ring, or -1 off the ring.
+__host__ __device__ __forceinline__ long ring_index(const KRing &r, int i, int j) {
+    const int L = r.L, N = r.N;
+    if (i < L - N || i >= r.Xn - L + N || j < L - N || j >= r.Zn - L + N) return -1;
+    if (i >= L + N && i < r.Xn - L - N && j >= L + N && j < r.Zn - L - N) return -1;
+    const int w = r.nz - 2 * N, wo = N - 1, wi = N + 1;
+    const long cw = r.nx + 2 * N;
+    if (j < L + N) {
+        const long c = i - (L - N);
+        const int m = j - (L - N);
+        return m < wo ? c * wo + m : r.nouter + c * wi + (m - wo);
+    }
+    if (j >= r.Zn - L - N) {
+        const long c = i - (L - N);
+        const int m = j - (r.Zn - L - N);
+        return m <= N ? r.nouter + cw * wi + c * wi + m : cw * wo + c * wo + (m - wi);
+    }
+    const long row = j - (L + N);
+    if (i < L + N) {
+        const int m = i - (L - N);
+        return m < wo ? 2 * cw * wo + (long)m * w + row : r.nouter + 2 * cw * wi + (long)(m - wo) * w + row;
+    }
+    const int m = i - (r.Xn - L - N);
+    return m <= N ? r.nouter + 2 * cw * wi + (long)wi * w + (long)m * w + row
+                  : 2 * cw * wo + (long)wo * w + (long)(m - wi) * w + row;
 }
 
 // Strip slot t holds, in this field order: vx, vz (V^{t+1}, the reference's
@@ -319,15 +367,37 @@ enum { RS_VX = 0, RS_VZ, RS_TH, RS_OM, RS_GZX, RS_GZZ, RS_GXX, RS_GXZ, RS_NF };
 // v^{s+1} - d^{s+1}, into a two-slot buffer; the FP32 rounding of that
 // subtraction is bounded by the keyframe spacing.  Slots are stored in time
 // order, with the encoder state zero before the first.
+//
+// The FP16 store keeps theta and the four gradients exactly (FP32) on the
+// inner part of the ring.  The reconstruction integrates the split source
+// velocities on the ring's interior rows open loop from these fields (their
+// stress pass never sees the split parts), so the rounding of those values
+// accumulates in the imaged source field; residual coding there raises the
+// image error on the rows next to the edge about fifteenfold over FP32 strips,
+// while FP32 storage of the inner part brings it back to the FP32 strips'
+// level.  Omega drives only the S part, which the imaging condition does not
+// read from the source side, and the velocities feed closed-loop stress, so
+// those three fields stay residual-coded on the whole ring.
 constexpr int kStripKey = 32;
+constexpr unsigned kInner32 = (1u << RS_TH) | (1u << RS_GZX) | (1u << RS_GZZ) | (1u << RS_GXX) | (1u << RS_GXZ);
+constexpr int kNInner32 = 5;
+__host__ __device__ constexpr bool inner32_field(int f) { return (kInner32 >> f) & 1u; }
+__host__ __device__ constexpr int inner32_rank(int f) { return __builtin_popcount(kInner32 & ((1u << f) - 1u)); }
+// FP16 elements before field f within one slot of the FP16 store: fields with
+// an FP32 inner part code only the ring's outer part.
+__host__ __device__ constexpr long strip16_offset(long ncells, long nouter, int f) {
+    return ncells * __builtin_popcount(~kInner32 & ((1u << f) - 1u)) + nouter * __builtin_popcount(kInner32 & ((1u << f) - 1u));
+}
+
 struct KStrips {
     float *f32;          // [slot][RS_NF][ncells] (FP32 strips)
-    __half *f16;         // [slot][RS_NF][ncells] residuals (FP16 strips)
+    __half *f16;         // [slot][strip16_offset(.., RS_NF)] residuals (FP16 strips), field f at strip16_offset(f)
+    float *in32;         // [slot][kNInner32][ncells - nouter] inner ring part of the FP32-kept fields (FP16 strips)
     int8_t *e8;          // [slot][RS_NF][nchunks] chunk exponents
     float *key;          // [slot / kStripKey][RS_NF][ncells] keyframes
     float *err;          // [2][RS_NF][ncells] encoder state: previous decoded value, fed-back error
     float *dec;          // [2][RS_NF][ncells] decoded slots, by slot parity
-    long ncells, nchunks;
+    long ncells, nchunks, nouter;
     int fp16;
     int last_v, last_th; // last slot stored for the two velocity fields / the six stress-like fields
 };
@@ -338,11 +408,13 @@ struct KStripStride {
     long chunks; // exponent entries
     long key;    // keyframe elements
     long aux;    // encoder-state and decoded-slot elements
+    long in32;   // inner FP32 elements
 };
 
 __device__ __forceinline__ KStrips strips_of_shot(KStrips s, const KStripStride &d, long z) {
     if (s.f32) s.f32 += z * d.strip;
     if (s.f16) s.f16 += z * d.strip;
+    if (s.in32) s.in32 += z * d.in32;
     if (s.e8) s.e8 += z * d.chunks;
     if (s.key) s.key += z * d.key;
     if (s.err) s.err += z * d.aux;
@@ -368,21 +440,25 @@ __device__ __forceinline__ bool strip_keyframe(const KStrips &s, int slot, int f
     return slot % kStripKey == kStripKey - 1 || slot == (f < RS_TH ? s.last_v : s.last_th);
 }
 
-// Store `v` of ring cell q (one lane per cell, 64 consecutive cells per wave).
+// Store `v` of ring cell q (one lane per cell, 64 consecutive cells per wave;
+// every lane of the wave takes part, valid or not).
 __device__ __forceinline__ void strip_put(const KStrips &s, int slot, int f, long q, bool valid, float v) {
-    const long base = ((long)slot * RS_NF + f) * s.ncells;
     if (!s.fp16) {
-        if (valid) s.f32[base + q] = v;
+        if (valid) s.f32[((long)slot * RS_NF + f) * s.ncells + q] = v;
         return;
     }
+    const bool kept = inner32_field(f) && q >= s.nouter;
+    if (valid && kept) s.in32[((long)slot * kNInner32 + inner32_rank(f)) * (s.ncells - s.nouter) + (q - s.nouter)] = v;
+    const bool coded = valid && !kept;
+    const long base = (long)slot * strip16_offset(s.ncells, s.nouter, RS_NF) + strip16_offset(s.ncells, s.nouter, f);
     const long fc = (long)f * s.ncells + q;
     float *const pp = s.err + fc, *const rp = s.err + (long)RS_NF * s.ncells + fc;
-    const float pred = valid ? *pp : 0.f, fb = valid ? *rp : 0.f;
+    const float pred = coded ? *pp : 0.f, fb = coded ? *rp : 0.f;
     const float x = (v - pred) + fb;
-    float mx = wave_max(valid ? fabsf(x) : 0.f);
+    float mx = wave_max(coded ? fabsf(x) : 0.f);
     int e = scale_exponent(mx);
     const __half h = __float2half_rn(ldexpf(x, -e));
-    if (valid) {
+    if (coded) {
         s.f16[base + q] = h;
         const float d = ldexpf(__half2float(h), e);
         const float r = x - d; // exact: d is x rounded to 11 bits
@@ -417,11 +493,14 @@ static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slo
     const int last = f < RS_TH ? s.last_v : s.last_th;
     float v = 0.f;
     if (slot >= 0 && slot <= last) {
-        if (strip_keyframe(s, slot, f)) {
+        if (inner32_field(f) && q >= s.nouter) {
+            v = s.in32[((long)slot * kNInner32 + inner32_rank(f)) * (s.ncells - s.nouter) + (q - s.nouter)];
+        } else if (strip_keyframe(s, slot, f)) {
             v = key[((long)(slot / kStripKey) * RS_NF + f) * s.ncells + q];
         } else {
-            const long b1 = (long)(slot + 1) * RS_NF + f;
-            const float d = ldexpf(__half2float(f16[b1 * s.ncells + q]), e8[b1 * s.nchunks + (q >> 6)]);
+            const long b1 = (long)(slot + 1) * strip16_offset(s.ncells, s.nouter, RS_NF) +
+                            strip16_offset(s.ncells, s.nouter, f);
+            const float d = ldexpf(__half2float(f16[b1 + q]), e8[((long)(slot + 1) * RS_NF + f) * s.nchunks + (q >> 6)]);
             v = dec[((long)((slot + 1) & 1) * RS_NF + f) * s.ncells + q] - d;
         }
     }

@@ -125,6 +125,7 @@ RingDesc make_ring(const Grid &g) {
     r.ntop = (int64_t)(g.nx + 2 * g.N) * (2 * g.N);
     r.nside = (int64_t)(2 * g.N) * (g.nz - 2 * g.N);
     r.ncells = 2 * r.ntop + 2 * r.nside;
+    r.nouter = (int64_t)(g.N - 1) * (2 * (g.nx + 2 * g.N) + 2 * (g.nz - 2 * g.N));
     return r;
 }
 

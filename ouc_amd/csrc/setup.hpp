@@ -53,10 +53,11 @@ void mute_counts(const Grid &g, const ouc_shot &s, float vp_src, std::vector<int
 // edge, as four rectangles (top, bottom: (nx+2N) x 2N; left, right:
 // 2N x (nz-2N)).  These are exactly the cells the interior-only source
 // reconstruction reads outside its own update, so they replace the reference's
-// full-length 2N-wide strips (P-S-seperating.cu:1230-1287).
+// full-length 2N-wide strips (P-S-seperating.cu:1230-1287).  nouter counts the
+// cells of the N-1 outermost pad rows (columns) of each rectangle, stored first.
 struct RingDesc {
     int L, N, Xn, Zn, nx, nz;
-    int64_t ntop, nside, ncells;
+    int64_t ntop, nside, ncells, nouter;
 };
 RingDesc make_ring(const Grid &g);
 
