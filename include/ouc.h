@@ -362,6 +362,17 @@ ouc_status ouc_recon_snapshots(ouc_ctx *c, const ouc_shot *s, int32_t n, const i
  * gradient is taken with respect to them. */
 ouc_status ouc_lb_forward(int32_t device, int32_t M, int32_t H, int32_t W, const float *x, const float *weights,
                           float *out);
+/* ouc_lb_forward without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  x, weights and out are DEVICE pointers with the
+ * layouts of ouc_lb_forward; the BF16 weight copy and the hidden activations
+ * live in a static per-GPU scratch array (16 MiB + 8 KiB, held until the
+ * process ends), so M*H*W is at most 131072 (else OUC_INVALID_ARG).  The
+ * launches are queued on `stream` (a hipStream_t; NULL for the null stream)
+ * and the call returns without waiting.  Calls that overlap on one GPU share
+ * the scratch and race: order them on one stream. */
+ouc_status ouc_lb_forward_device(int32_t device, int32_t M, int32_t H, int32_t W, const float *x,
+                                 const float *weights, float *out, void *stream);
 ouc_status ouc_lb_loss_grad(int32_t device, int32_t M, int32_t H, int32_t W, const float *x, const float *y,
                             const float *mask, const float *weights, float *loss, float *grad);
 /* Adam (beta 0.9, 0.999, eps 1e-8) over `epochs` full-batch steps; weights

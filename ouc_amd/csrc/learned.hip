@@ -14,6 +14,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -147,13 +149,30 @@ __global__ void k_adam(int n, float *__restrict__ w, const float *__restrict__ g
     w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+constexpr int kNW = kWSizes[0] + kWSizes[1] + kWSizes[2];
+
+// The forward pass: BF16 rounding of the FP32 master weights w32 into wb16,
+// then the three convolutions act[0] -> act[1] -> act[2] -> act[3].
+void forward_layers(hipStream_t s, int M, int H, int W, const float *w32, __hip_bfloat16 *wb16,
+                    const float *const act[4]) {
+    const long HW = (long)H * W;
+    const int off[3] = {0, kWSizes[0], kWSizes[0] + kWSizes[1]};
+    k_round_bf16<<<(kNW + 255) / 256, 256, 0, s>>>(kNW, w32, wb16);
+    for (int l = 0; l < 3; ++l) {
+        long tot = (long)M * kC[l + 1] * HW;
+        k_conv_fwd<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(M, kC[l], kC[l + 1], H, W, act[l], wb16 + off[l],
+                                                               (float *)act[l + 1]);
+    }
+    OUC_HIP(hipGetLastError());
+}
+
 struct Net {
     int M = 0, H = 0, W = 0;
     DevBuf w32, wb16, g32, m1, m2;      // all layers concatenated
     DevBuf act[4], dact[4];             // activations and their gradients
     DevBuf x, y, mask, dout, partial;
     int off[3] = {0, kWSizes[0], kWSizes[0] + kWSizes[1]};
-    int nw = kWSizes[0] + kWSizes[1] + kWSizes[2];
+    int nw = kNW;
 
     void alloc(int M_, int H_, int W_) {
         M = M_; H = H_; W = W_;
@@ -170,14 +189,8 @@ struct Net {
     __hip_bfloat16 *wl(int l) const { return wb16.as<__hip_bfloat16>() + off[l]; }
 
     void forward(hipStream_t s) {
-        const long HW = (long)H * W;
-        k_round_bf16<<<(nw + 255) / 256, 256, 0, s>>>(nw, w32.f(), wb16.as<__hip_bfloat16>());
-        for (int l = 0; l < 3; ++l) {
-            long tot = (long)M * kC[l + 1] * HW;
-            k_conv_fwd<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(M, kC[l], kC[l + 1], H, W, act[l].f(), wl(l),
-                                                                   act[l + 1].f());
-        }
-        OUC_HIP(hipGetLastError());
+        const float *acts[4] = {act[0].f(), act[1].f(), act[2].f(), act[3].f()};
+        forward_layers(s, M, H, W, w32.f(), wb16.as<__hip_bfloat16>(), acts);
     }
 
     // forward + loss + backward; returns the loss
@@ -215,6 +228,36 @@ float mask_total(const float *mask, long n) {
     return (float)s;
 }
 
+// Scratch of the allocation-free forward pass (ouc_lb_forward_device): the
+// BF16 copy of the weights in the first kLbWeightBytes, then the two hidden
+// activation maps [M][16][H][W] of at most kLbMaxPixels = M*H*W pixels each.
+// A file-scope device array: every GPU the library runs on holds its own copy
+// (16 MiB + 8 KiB) until the process ends, and every call on that GPU shares it.
+constexpr long kLbMaxPixels = 131072;
+constexpr size_t kLbWeightBytes = 8192;
+static_assert(kLbWeightBytes >= kNW * sizeof(__hip_bfloat16), "weight part of the scratch too small");
+constexpr size_t kLbScratchBytes = kLbWeightBytes + 2 * sizeof(float) * 16 * (size_t)kLbMaxPixels;
+static __device__ __align__(256) unsigned char scratch[kLbScratchBytes];
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current.  The array's name is a host-side stand-in in host code;
+// only these addresses are passed to kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+void *scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(OUC_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        OUC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(OUC_DEVICE, "scratch has no device address");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
 } // namespace
 
 extern "C" {
@@ -233,6 +276,31 @@ ouc_status ouc_lb_forward(int32_t device, int32_t M, int32_t H, int32_t W, const
         OUC_HIP(hipMemcpy(net.w32.ptr, weights, net.nw * 4, hipMemcpyHostToDevice));
         net.forward(nullptr);
         OUC_HIP(hipMemcpy(out, net.act[3].ptr, (size_t)M * 2 * HW * 4, hipMemcpyDeviceToHost));
+        return OUC_OK;
+    } catch (const Error &e) {
+        return lb_fail(e.what(), e.code);
+    }
+}
+
+// ouc_lb_forward on device buffers, with no allocation: the weight copy and
+// the hidden activations live in the per-GPU `scratch` array; the four
+// launches are queued on `stream` (NULL: the null stream) and the call returns
+// without waiting for them.
+ouc_status ouc_lb_forward_device(int32_t device, int32_t M, int32_t H, int32_t W, const float *x,
+                                 const float *weights, float *out, void *stream) {
+    if (M < 1 || H < 1 || W < 1 || !x || !weights || !out) return lb_fail("bad arguments", OUC_INVALID_ARG);
+    if ((long)M * H * W > kLbMaxPixels)
+        return lb_fail("M*H*W = " + std::to_string((long)M * H * W) + " exceeds the scratch maximum of " +
+                           std::to_string(kLbMaxPixels) + " pixels",
+                       OUC_INVALID_ARG);
+    try {
+        OUC_HIP(hipSetDevice(device));
+        unsigned char *base = (unsigned char *)scratch_on(device);
+        __hip_bfloat16 *wb16 = (__hip_bfloat16 *)base;
+        float *h1 = (float *)(base + kLbWeightBytes);
+        float *h2 = h1 + (size_t)16 * M * H * W;
+        const float *acts[4] = {x, h1, h2, out};
+        forward_layers((hipStream_t)stream, M, H, W, weights, wb16, acts);
         return OUC_OK;
     } catch (const Error &e) {
         return lb_fail(e.what(), e.code);

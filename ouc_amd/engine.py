@@ -87,6 +87,9 @@ def lib():
     L.ouc_recon_snapshots.restype = C.c_int
     L.ouc_lb_forward.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp]
     L.ouc_lb_forward.restype = C.c_int
+    L.ouc_lb_forward_device.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
+    L.ouc_lb_forward_device.restype = C.c_int
     L.ouc_lb_loss_grad.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp, fp, fp, fp]
     L.ouc_lb_loss_grad.restype = C.c_int
     L.ouc_lb_train.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, fp, fp, C.c_int32, C.c_float, fp, fp]
@@ -384,6 +387,15 @@ def lb_forward(x, weights, device=0):
     out = np.zeros((M, 2, H, W), np.float32)
     _lb_check(lib().ouc_lb_forward(int(device), M, H, W, abi.fptr(x), abi.fptr(w), abi.fptr(out)))
     return out
+
+
+def lb_forward_device(M, H, W, x_ptr, weights_ptr, out_ptr, device=0, stream=None):
+    """ouc_lb_forward_device on device addresses (integers, e.g. a torch
+    tensor's data_ptr()): queues the forward pass on `stream` (a hipStream_t
+    handle, None for the null stream) without allocating or waiting."""
+    _lb_check(lib().ouc_lb_forward_device(int(device), int(M), int(H), int(W), C.c_void_p(x_ptr),
+                                          C.c_void_p(weights_ptr), C.c_void_p(out_ptr),
+                                          C.c_void_p(stream) if stream else None))
 
 
 def lb_loss_grad(x, y, mask, weights, device=0):
