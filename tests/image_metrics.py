@@ -1,0 +1,102 @@
+"""Relative-L2 image errors over the regions the image tests hold to the bar.
+
+Regions (cells where both images are finite; the reference's num/den gives
+NaN where den underflows):
+  whole   - the whole interior;
+  below8  - rows 8 and deeper;
+  near    - rows 8 and deeper within 512 cells of the source in x;
+  top     - rows 0..7, the receiver row and its neighbours (rows 2..7 for the
+            Laplacians, whose 2-cell border the reference never writes,
+            P-S-seperating.cu:1457-1469).
+"""
+import numpy as np
+
+IMAGE_BAR = 1e-4  # relative L2 against FP64 (images at FP32)
+FIELD_BAR = 1e-5  # relative L2 against FP64 (seismograms at FP32)
+REGIONS = ("whole", "below8", "near", "top")
+KINDS = ("img_pp", "img_ps", "lap_pp", "lap_ps")
+NEAR_CELLS = 512
+
+
+def rel(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = np.isfinite(a) & np.isfinite(b)
+    if mask is not None:
+        m &= mask
+    return float(np.linalg.norm((a - b)[m]) / np.linalg.norm(b[m]))
+
+
+def laplace5(img, dh):
+    """The reference's Laplace (5-point / dh^2, zero 2-cell border)."""
+    img = np.asarray(img, np.float64)
+    out = np.zeros_like(img)
+    c = img[2:-2, 2:-2]
+    out[2:-2, 2:-2] = (img[3:-1, 2:-2] + img[1:-3, 2:-2] + img[2:-2, 3:-1] + img[2:-2, 1:-3] - 4 * c) / dh ** 2
+    return out
+
+
+def region_mask(shape, region, sx, x_step=1, z_step=1, kind="img_pp"):
+    """Mask over an image sampled at x = 0, x_step, ..., z = 0, z_step, ..."""
+    x = np.arange(shape[0])[:, None] * x_step
+    z = np.arange(shape[1])[None, :] * z_step
+    lap = kind.startswith("lap")
+    if region == "whole":
+        m = np.ones(shape, bool)
+        if lap:
+            m &= (z >= 2) & (x >= 2) & (x < shape[0] * x_step - 2)
+        return m
+    if region == "below8":
+        return np.broadcast_to(z >= 8, shape).copy()
+    if region == "near":
+        return (z >= 8) & (np.abs(x - sx) < NEAR_CELLS)
+    if region == "top":
+        m = np.broadcast_to(z < 8, shape).copy()
+        if lap:
+            m &= (z >= 2) & (x >= 2) & (x < shape[0] * x_step - 2)
+        return m
+    raise ValueError(region)
+
+
+def errors_full(gpu, ref, sx):
+    """Errors of full-resolution GPU images against full-resolution FP64
+    images (dicts with img_pp, img_ps, lap_pp, lap_ps)."""
+    out = {}
+    for k in KINDS:
+        for r in REGIONS:
+            out[(k, r)] = rel(gpu[k], ref[k], region_mask(ref[k].shape, r, sx, kind=k))
+    return out
+
+
+def errors_fixture(gpu, fix, shot, dh):
+    """Errors of full-resolution GPU images against a decimated FP64 fixture
+    (tests/golden/make_config_fixtures.py): the decimated grid for whole,
+    below8 and near; the fixture's full-resolution top rows for top."""
+    ddx, ddz = (int(v) for v in fix["decimation"])
+    ntop = int(fix["top_rows"])
+    sx = int(fix[f"shot{shot}_sx"])
+    out = {}
+    for k in KINDS:
+        a = np.asarray(gpu[k], np.float64)
+        ref = fix[f"shot{shot}_{k}"]
+        for r in ("whole", "below8", "near"):
+            out[(k, r)] = rel(a[::ddx, ::ddz], ref, region_mask(ref.shape, r, sx, ddx, ddz, kind=k))
+        if k.startswith("img"):
+            top = fix[f"shot{shot}_top_{k[4:]}"]
+            out[(k, "top")] = rel(a[:, :8], top[:, :8], region_mask(top[:, :8].shape, "top", sx, kind=k))
+        else:
+            top = laplace5(np.nan_to_num(fix[f"shot{shot}_top_{k[4:]}"].astype(np.float64)), dh)
+            m = region_mask(top[:, :8].shape, "top", sx, kind=k)
+            out[(k, "top")] = rel(a[:, :8], top[:, :8], m)
+    return out
+
+
+def bar_param(key, misses, bar, *values, id=None):
+    """A pytest.param for one (quantity, region) case, marked
+    xfail(strict=True) with the measured error when `misses` lists `key`."""
+    import pytest
+
+    marks = []
+    if key in misses:
+        marks = [pytest.mark.xfail(strict=True, reason=f"measured {misses[key]:.1e} > bar {bar:.0e}")]
+    return pytest.param(*values, marks=marks, id=id)

@@ -28,6 +28,40 @@ def test_model_files_match_reference_layout(tmp_path):
         assert raw == a.astype("<f4").tobytes()  # for i in x: for j in z: fwrite(float)
 
 
+def test_image_files_match_reference_layout(tmp_path):
+    """Per-shot Laplacian images and the stack under the reference's names
+    (image_PP_lap_<nx>_<nz>_<l>.dat, All_image_PP_lap.dat, ...), each the
+    interior written float by float with x outer and z inner
+    (P-S-seperating.cu:2379-2400, 2446-2467).  Reading the per-shot files back
+    and summing them, as the reference's stack loop does
+    (P-S-seperating.cu:2403-2443), gives the stack file's values."""
+    from ouc_amd.engine import write_image_files
+
+    rng = np.random.default_rng(2)
+    nx, nz = 37, 23
+    shots = {l: (rng.standard_normal((nx, nz)).astype(np.float32), rng.standard_normal((nx, nz)).astype(np.float32))
+             for l in (0, 3)}
+    out = str(tmp_path / "image2")
+    for l, (pp, ps) in shots.items():
+        write_image_files(out, pp, ps, l)
+    stack = [np.zeros((nx, nz), np.float32), np.zeros((nx, nz), np.float32)]
+    for l in sorted(shots):
+        for a, tag in enumerate(("PP", "PS")):
+            raw = open(os.path.join(out, f"image_{tag}_lap_{nx}_{nz}_{l}.dat"), "rb").read()
+            assert len(raw) == 4 * nx * nz
+            vals = struct.unpack(f"<{nx * nz}f", raw)
+            img = np.array(vals, np.float32).reshape(nx, nz)  # for i in x: for j in z
+            assert np.array_equal(img, shots[l][a])
+            stack[a] += img
+    write_image_files(out, stack[0], stack[1], -1)
+    for a, tag in enumerate(("PP", "PS")):
+        raw = open(os.path.join(out, f"All_image_{tag}_lap.dat"), "rb").read()
+        assert raw == stack[a].astype("<f4").tobytes()
+    assert sorted(os.listdir(out)) == sorted(
+        [f"image_{t}_lap_{nx}_{nz}_{l}.dat" for t in ("PP", "PS") for l in shots]
+        + ["All_image_PP_lap.dat", "All_image_PS_lap.dat"])
+
+
 def _be16(b, pos):  # 1-based byte position, as the standard numbers them
     return struct.unpack(">h", b[pos - 1:pos + 1])[0]
 

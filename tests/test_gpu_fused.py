@@ -1,9 +1,11 @@
-"""GPU parity of the fused engine (batched single-pass kernels) against the
-FP64 oracle and against the plain engine."""
+"""The fused engine (batched single-pass kernels): forward seismograms against
+the FP64 oracle, batches against single shots, graph replays against eager
+launches.  Its images are held to the FP64 oracle in test_gpu_parity.py."""
 import numpy as np
 import pytest
 
 from conftest import rel_l2
+from image_metrics import FIELD_BAR, bar_param
 from ouc_amd import abi, velocity_models as vm
 from ouc_amd.engine import Engine
 
@@ -14,35 +16,6 @@ def _case(semantics=abi.SEM_FAITHFUL, engine=abi.ENGINE_FUSED, nx=160, nz=96, nt
     vp, vs, _ = vm.layered_random(nx, nz, seed=3)
     p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, semantics=semantics, engine=engine, **kw)
     return p, vp, vs
-
-
-@pytest.mark.parametrize("semantics", [abi.SEM_FAITHFUL, abi.SEM_EXACT])
-def test_fused_rtm_vs_oracle(oracle, semantics):
-    p, vp, vs = _case(semantics)
-    shot = dict(sx=80, sz=0, rec_z=0, fm=22.0, mute=1)
-    print(f"[case] fused rtm 160x96 nt=900 semantics={semantics}", flush=True)
-    with Engine(p, vp, vs) as e:
-        img = e.rtm_shot(shot)
-    ref = oracle.rtm_shot(p, vp, vs, shot, 64)
-    errs = {k: rel_l2(img[k], ref[k]) for k in ["img_pp", "img_ps", "lap_pp", "lap_ps"]}
-    deep = {k: rel_l2(img[k][:, 8:], ref[k][:, 8:]) for k in ["img_pp", "img_ps"]}
-    print("[fused rtm] rel-L2 vs FP64:", " ".join(f"{k}={v:.2e}" for k, v in errs.items()),
-          " below row 8:", " ".join(f"{k}={v:.2e}" for k, v in deep.items()), flush=True)
-    for k in deep:
-        assert deep[k] < 1e-3, (k, deep[k])
-
-
-def test_fused_agrees_with_plain(oracle):
-    shot = dict(sx=70, sz=0, rec_z=0, fm=22.0, mute=1)
-    p, vp, vs = _case()
-    with Engine(p, vp, vs) as e:
-        a = e.rtm_shot(shot)
-    p.engine = abi.ENGINE_PLAIN
-    with Engine(p, vp, vs) as e:
-        b = e.rtm_shot(shot)
-    d = rel_l2(a["img_pp"][:, 8:], b["img_pp"][:, 8:])
-    print(f"[fused vs plain] img_pp below row 8: {d:.2e}", flush=True)
-    assert d < 1e-3
 
 
 def test_fused_batch_is_sum_of_single_shots(oracle):
@@ -63,21 +36,29 @@ def test_fused_batch_is_sum_of_single_shots(oracle):
     assert np.array_equal(out["stack_ps"], sps, equal_nan=True)
 
 
-def test_fused_forward_seismograms_vs_oracle(oracle):
-    """The fused engine's forward path (ouc_forward_shot with ENGINE_FUSED):
-    the six seismogram components against the FP64 oracle."""
+FWD_MISSES = {
+    'vsx': 1.03e-04,
+    'vsz': 8.23e-05,
+}
+FWD_COMP = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
+_fwd = {}
+
+
+@pytest.mark.parametrize("comp", [bar_param(c, FWD_MISSES, FIELD_BAR, c, id=c) for c in FWD_COMP])
+def test_fused_forward_seismograms_vs_oracle(oracle, comp):
+    """The fused engine's forward path (ouc_forward_shot with ENGINE_FUSED),
+    160 x 96, 700 steps, source and receivers at depth 5: each of the six
+    seismogram components against the FP64 oracle at the 1e-5 field bar."""
     p, vp, vs = _case(nt=700)
     shot = dict(sx=80, sz=5, rec_z=5, fm=25.0)
-    with Engine(p, vp, vs) as e:
-        seis = e.forward_shot(shot)
-    ref = oracle.forward_shot(p, vp, vs, shot, 64)
-    names = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
-    errs = {n: rel_l2(seis[k], ref[k]) for k, n in enumerate(names)}
-    print("[fused forward] rel-L2 vs FP64:", " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
-    for n in ("vx", "vz", "vpx", "vpz"):
-        assert errs[n] < 1e-5, (n, errs[n])
-    for n in ("vsx", "vsz"):
-        assert errs[n] < 1e-3, (n, errs[n])
+    if not _fwd:
+        with Engine(p, vp, vs) as e:
+            _fwd["gpu"] = e.forward_shot(shot)
+        _fwd["ref"] = oracle.forward_shot(p, vp, vs, shot, 64)
+    q = FWD_COMP.index(comp)
+    err = rel_l2(_fwd["gpu"][q], _fwd["ref"][q])
+    print(f"[fused forward] {comp}: {err:.2e}", flush=True)
+    assert err < FIELD_BAR, err
 
 
 @pytest.mark.parametrize("learned", [False, True])

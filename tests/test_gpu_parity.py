@@ -1,97 +1,192 @@
 """GPU parity: the HIP engine through its C ABI against the FP64 oracle.
 
-Tolerances: relative L2 against the FP64 oracle.  Fields (seismograms) 1e-5,
-images 1e-4, the bars of the FP32 path; where a component's measured error is
-above its bar the test records the measured value and checks it against the
-looser bound stated next to it.
+Bars (relative L2 against the FP64 oracle, tests/image_metrics.py): fields
+(seismograms) 1e-5, images 1e-4.  Every component, and every image over every
+region, is a case of its own; a case the engine misses is marked
+xfail(strict=True) with its measured error (an improvement then shows as an
+unexpected pass), and the FP32 oracle's error on the same case (the
+reference's own arithmetic in FP32 on the CPU) is printed beside it.
 """
 import numpy as np
 import pytest
 
 from conftest import rel_l2
+from image_metrics import FIELD_BAR, IMAGE_BAR, KINDS, REGIONS, bar_param, errors_full, rel
 from ouc_amd import abi, velocity_models as vm
 from ouc_amd.engine import Engine
 
 pytestmark = pytest.mark.gpu
 
 COMP = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
+ENGINES = {"plain": abi.ENGINE_PLAIN, "fused": abi.ENGINE_FUSED}
+SEMANTICS = {"faithful": abi.SEM_FAITHFUL, "exact": abi.SEM_EXACT}
+
+# Measured errors of the cases that miss their bar.
+FIELD_MISSES = {
+    ('plain', 'vsx'): 7.94e-05,
+    ('plain', 'vsz'): 2.41e-04,
+    ('fused', 'vpx'): 8.05e-05,
+    ('fused', 'vpz'): 1.60e-04,
+    ('fused', 'vsx'): 2.50e-03,
+    ('fused', 'vsz'): 8.76e-03,
+}
+IMAGE_MISSES = {
+    ('plain', 'faithful', 'img_pp', 'whole'): 1.34e-03,
+    ('plain', 'faithful', 'img_pp', 'top'): 4.64e-03,
+    ('plain', 'faithful', 'img_ps', 'whole'): 1.01e-03,
+    ('plain', 'faithful', 'img_ps', 'top'): 6.44e-03,
+    ('plain', 'faithful', 'lap_pp', 'whole'): 7.47e-04,
+    ('plain', 'faithful', 'lap_pp', 'below8'): 1.73e-04,
+    ('plain', 'faithful', 'lap_pp', 'near'): 1.73e-04,
+    ('plain', 'faithful', 'lap_pp', 'top'): 9.65e-04,
+    ('plain', 'faithful', 'lap_ps', 'whole'): 5.47e-04,
+    ('plain', 'faithful', 'lap_ps', 'top'): 9.62e-04,
+    ('plain', 'exact', 'img_pp', 'whole'): 1.37e-03,
+    ('plain', 'exact', 'img_pp', 'top'): 5.63e-03,
+    ('plain', 'exact', 'img_ps', 'whole'): 1.03e-03,
+    ('plain', 'exact', 'img_ps', 'top'): 9.80e-03,
+    ('plain', 'exact', 'lap_pp', 'whole'): 1.53e-03,
+    ('plain', 'exact', 'lap_pp', 'below8'): 2.07e-04,
+    ('plain', 'exact', 'lap_pp', 'near'): 2.07e-04,
+    ('plain', 'exact', 'lap_pp', 'top'): 2.60e-02,
+    ('plain', 'exact', 'lap_ps', 'whole'): 7.12e-04,
+    ('plain', 'exact', 'lap_ps', 'top'): 6.83e-03,
+    ('fused', 'faithful', 'img_pp', 'whole'): 7.81e-04,
+    ('fused', 'faithful', 'img_pp', 'top'): 2.70e-03,
+    ('fused', 'faithful', 'img_ps', 'whole'): 4.92e-04,
+    ('fused', 'faithful', 'img_ps', 'top'): 3.12e-03,
+    ('fused', 'faithful', 'lap_pp', 'whole'): 4.77e-04,
+    ('fused', 'faithful', 'lap_pp', 'below8'): 1.75e-04,
+    ('fused', 'faithful', 'lap_pp', 'near'): 1.75e-04,
+    ('fused', 'faithful', 'lap_pp', 'top'): 6.06e-04,
+    ('fused', 'faithful', 'lap_ps', 'whole'): 3.51e-04,
+    ('fused', 'faithful', 'lap_ps', 'top'): 6.03e-04,
+    ('fused', 'exact', 'img_pp', 'whole'): 7.83e-04,
+    ('fused', 'exact', 'img_pp', 'top'): 3.23e-03,
+    ('fused', 'exact', 'img_ps', 'whole'): 4.93e-04,
+    ('fused', 'exact', 'img_ps', 'top'): 4.70e-03,
+    ('fused', 'exact', 'lap_pp', 'whole'): 7.09e-04,
+    ('fused', 'exact', 'lap_pp', 'below8'): 2.29e-04,
+    ('fused', 'exact', 'lap_pp', 'near'): 2.29e-04,
+    ('fused', 'exact', 'lap_pp', 'top'): 1.15e-02,
+    ('fused', 'exact', 'lap_ps', 'whole'): 3.30e-04,
+    ('fused', 'exact', 'lap_ps', 'below8'): 1.03e-04,
+    ('fused', 'exact', 'lap_ps', 'near'): 1.03e-04,
+    ('fused', 'exact', 'lap_ps', 'top'): 3.02e-03,
+}
+FP16_MISSES = {
+    ('plain', 'img_pp', 'whole'): 1.35e-03,
+    ('plain', 'img_pp', 'top'): 4.67e-03,
+    ('plain', 'img_ps', 'whole'): 1.02e-03,
+    ('plain', 'img_ps', 'top'): 6.46e-03,
+    ('fused', 'img_pp', 'whole'): 6.47e-04,
+    ('fused', 'img_pp', 'top'): 2.24e-03,
+    ('fused', 'img_ps', 'whole'): 7.90e-04,
+    ('fused', 'img_ps', 'top'): 5.02e-03,
+}
+
+_cache = {}
 
 
-@pytest.fixture(scope="module")
-def cfg1_refs(oracle):
-    """The 200 x 200 two-layer forward case (2000 steps, 20-cell CPML at
-    R = 1e-4) in FP64 (the oracle) and in FP32 (the reference's own arithmetic
-    on the CPU, to which the split parts are ill-conditioned)."""
-    p, vp, vs, shot = vm.cfg_forward_200()
-    r64 = oracle.forward_shot(p, vp, vs, shot, 64)
-    r32 = oracle.forward_shot(p, vp, vs, shot, 32)
-    e32 = {c: rel_l2(r32[q], r64[q]) for q, c in enumerate(COMP)}
-    print("[forward 200] FP32 oracle rel-L2 vs FP64:", " ".join(f"{c}={v:.2e}" for c, v in e32.items()), flush=True)
-    return r64, e32
+def _memo(key, fn):
+    if key not in _cache:
+        _cache[key] = fn()
+    return _cache[key]
 
 
-@pytest.mark.parametrize("engine", [abi.ENGINE_PLAIN, abi.ENGINE_FUSED])
-def test_forward_two_layer_200(cfg1_refs, engine):
-    """Full-size forward modelling case: totals within the 1e-5 field bar; the
-    P/S split parts (formed by differences of large, nearly equal terms) are
-    reported and held below the FP32 oracle's own error: 10x below for the
-    plain engine (potential form: the curl never sees the gradient part), 2x
-    for the fused engine (total V and the S part: omega = q curl(V) carries the
-    rounding of the P part; measured vpx 8e-5, vpz 1.6e-4, vsx 2.5e-3, vsz
-    8.8e-3 against 9.5e-4, 7.7e-4, 3.0e-2, 4.2e-2 for the FP32 oracle)."""
-    p, vp, vs, shot = vm.cfg_forward_200()
-    p.engine = engine
-    name = "plain" if engine == abi.ENGINE_PLAIN else "fused"
-    print(f"[case] forward 200x200 N=5 nt={p.nt} {name}", flush=True)
-    with Engine(p, vp, vs) as e:
-        seis = e.forward_shot(shot)
-    ref, e32 = cfg1_refs
-    errs = {c: rel_l2(seis[q], ref[q]) for q, c in enumerate(COMP)}
-    print(f"[forward 200 {name}] rel-L2 vs FP64:", " ".join(f"{c}={v:.2e}" for c, v in errs.items()), flush=True)
-    for c in ["vx", "vz"]:
-        assert errs[c] < 1e-5, (c, errs[c])
-    plain = engine == abi.ENGINE_PLAIN
-    for c in ["vpx", "vpz"]:
-        assert errs[c] < (1e-5 if plain else 1e-3), (c, errs[c])
-    for c in ["vpx", "vpz", "vsx", "vsz"]:
-        assert errs[c] < (0.1 if plain else 0.5) * e32[c], (c, errs[c], e32[c])
+# ---- cfg1: 200 x 200 two-layer forward modelling, 2000 steps ---------------
+
+def _cfg1_ref(oracle):
+    def run():
+        p, vp, vs, shot = vm.cfg_forward_200()
+        r64 = oracle.forward_shot(p, vp, vs, shot, 64)
+        r32 = oracle.forward_shot(p, vp, vs, shot, 32)
+        e32 = {c: rel_l2(r32[q], r64[q]) for q, c in enumerate(COMP)}
+        print("[forward 200] FP32 oracle vs FP64:", " ".join(f"{c}={v:.2e}" for c, v in e32.items()), flush=True)
+        return r64, e32
+    return _memo("cfg1_ref", run)
 
 
-@pytest.mark.parametrize("semantics", [abi.SEM_FAITHFUL, abi.SEM_EXACT])
-def test_rtm_small(oracle, semantics):
+def _cfg1_gpu(engine):
+    def run():
+        p, vp, vs, shot = vm.cfg_forward_200()
+        p.engine = ENGINES[engine]
+        print(f"[case] forward 200x200 N=5 nt={p.nt} {engine}", flush=True)
+        with Engine(p, vp, vs) as e:
+            return e.forward_shot(shot)
+    return _memo(("cfg1", engine), run)
+
+
+@pytest.mark.parametrize("engine,comp", [bar_param((e, c), FIELD_MISSES, FIELD_BAR, e, c, id=f"{e}-{c}")
+                                         for e in ENGINES for c in COMP])
+def test_forward_two_layer_200(oracle, engine, comp):
+    """Full-size forward modelling case (200 x 200, 20-cell CPML at R = 1e-4,
+    2000 steps): each seismogram component within the 1e-5 field bar."""
+    ref, e32 = _cfg1_ref(oracle)
+    q = COMP.index(comp)
+    err = rel_l2(_cfg1_gpu(engine)[q], ref[q])
+    print(f"[forward 200 {engine}] {comp}: {err:.2e} (FP32 oracle {e32[comp]:.2e})", flush=True)
+    assert err < FIELD_BAR, err
+
+
+# ---- small RTM: 160 x 96, 900 steps ----------------------------------------
+
+def _small_case(semantics="faithful", engine="plain", strips=abi.STRIP_FP32):
     nx, nz, nt = 160, 96, 900
     vp, vs, _ = vm.layered_random(nx, nz, seed=3)
-    p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, semantics=semantics)
+    p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, semantics=SEMANTICS[semantics],
+                        engine=ENGINES[engine], strip_precision=strips)
     shot = dict(sx=80, sz=0, rec_z=0, fm=22.0, mute=1)
-    print(f"[case] rtm {nx}x{nz} nt={nt} semantics={semantics}", flush=True)
-    with Engine(p, vp, vs) as e:
-        img = e.rtm_shot(shot)
-    ref = oracle.rtm_shot(p, vp, vs, shot, 64)
-    errs = {k: rel_l2(img[k], ref[k]) for k in ["img_pp", "img_ps", "lap_pp", "lap_ps"]}
-    deep = {k: rel_l2(img[k][:, 8:], ref[k][:, 8:]) for k in ["img_pp", "img_ps"]}
-    print("[rtm small] rel-L2 vs FP64:", " ".join(f"{k}={v:.2e}" for k, v in errs.items()),
-          " below row 8:", " ".join(f"{k}={v:.2e}" for k, v in deep.items()), flush=True)
-    for k in deep:
-        assert deep[k] < 1e-3, (k, deep[k])
+    return p, vp, vs, shot
 
 
-@pytest.mark.parametrize("lap", [abi.LAP_FFT5])
-def test_fft_laplacian_equals_stencil(oracle, lap):
-    """rocFFT filter with the 5-point symbol equals the reference's stencil."""
-    nx, nz, nt = 120, 80, 700
-    vp, vs, _ = vm.layered_random(nx, nz, seed=4)
-    p = abi.make_params(nx, nz, 20, 5, nt, 5.0, 5.0, 0.0004, R=1e-3)
-    shot = dict(sx=60, sz=0, rec_z=0, fm=25.0, mute=1)
-    with Engine(p, vp, vs) as e:
-        a = e.rtm_shot(shot)
+def _small_ref(oracle, semantics, precision=64, strips=abi.STRIP_FP32):
+    def run():
+        p, vp, vs, shot = _small_case(semantics, strips=strips)
+        return oracle.rtm_shot(p, vp, vs, shot, precision)
+    return _memo(("small_ref", semantics, precision, strips), run)
+
+
+def _small_errors(oracle, engine, semantics, strips=abi.STRIP_FP32):
+    def run():
+        p, vp, vs, shot = _small_case(semantics, engine, strips)
+        print(f"[case] rtm 160x96 nt=900 {engine} {semantics} strips={strips}", flush=True)
+        with Engine(p, vp, vs) as e:
+            img = e.rtm_shot(shot)
+        errs = errors_full(img, _small_ref(oracle, semantics), shot["sx"])
+        return img, errs
+    return _memo(("small", engine, semantics, strips), run)
+
+
+@pytest.mark.parametrize("engine,semantics,kind,region", [
+    bar_param((e, s, k, r), IMAGE_MISSES, IMAGE_BAR, e, s, k, r, id=f"{e}-{s}-{k}-{r}")
+    for e in ENGINES for s in SEMANTICS for k in KINDS for r in REGIONS])
+def test_rtm_small(oracle, engine, semantics, kind, region):
+    _, errs = _small_errors(oracle, engine, semantics)
+    err = errs[(kind, region)]
+    p, vp, vs, shot = _small_case(semantics)
+    e32 = _memo(("small32", semantics), lambda: errors_full(_small_ref(oracle, semantics, 32),
+                                                             _small_ref(oracle, semantics), shot["sx"]))
+    print(f"[rtm small {engine} {semantics}] {kind} {region}: {err:.2e} (FP32 oracle {e32[(kind, region)]:.2e})",
+          flush=True)
+    assert err < IMAGE_BAR, err
+
+
+@pytest.mark.parametrize("lap", [abi.LAP_STENCIL, abi.LAP_FFT5])
+def test_laplacian_filters_against_the_oracle_filter(oracle, lap):
+    """The engine's Laplacian outputs (5-point stencil, and the same symbol as
+    a rocFFT filter) against the oracle's Laplace (P-S-seperating.cu:1457-1469)
+    applied in FP64 to the engine's own image, so the filter is judged apart
+    from the image it filters."""
+    p, vp, vs, shot = _small_case()
     p.laplacian = lap
     with Engine(p, vp, vs) as e:
-        b = e.rtm_shot(shot)
-    for k in ["lap_pp", "lap_ps"]:
-        err = rel_l2(b[k], a[k])
-        print(f"[fft laplacian] {k} FFT vs stencil rel-L2 {err:.2e}", flush=True)
-        assert err < 1e-5, (k, err)
-    assert np.array_equal(a["img_pp"], b["img_pp"])
+        img = e.rtm_shot(shot)
+    for k in ("pp", "ps"):
+        ref = oracle.laplace(img[f"img_{k}"].astype(np.float64), p.dx)
+        err = rel(img[f"lap_{k}"], ref)
+        print(f"[laplacian {lap}] lap_{k} against the oracle's filter: {err:.2e}", flush=True)
+        assert err < FIELD_BAR, (k, err)
 
 
 def test_fp16_codec_bit_exact(oracle):
@@ -121,23 +216,43 @@ def test_fp16_codec_bit_exact(oracle):
     assert np.array_equal(d.view(np.uint32), d0.view(np.uint32))  # backward decoding, keyframes every 32 slots
 
 
-@pytest.mark.parametrize("engine", [abi.ENGINE_PLAIN, abi.ENGINE_FUSED])
-def test_fp16_strips_rtm(oracle, engine):
-    """Predictive scaled-FP16 strips: the image error against the FP64 oracle
-    (exact strips) stays at the FP32-strip level, about 4e-5 below row 8 here
-    (the FP64 oracle with the same codec shows about 3.6e-5; rounding every
-    slot on its own gave 3e-2)."""
-    nx, nz, nt = 160, 96, 900
+@pytest.mark.parametrize("engine,kind,region", [
+    bar_param((e, k, r), FP16_MISSES, IMAGE_BAR, e, k, r, id=f"{e}-{k}-{r}")
+    for e in ENGINES for k in ("img_pp", "img_ps") for r in REGIONS])
+def test_fp16_strips_rtm(oracle, engine, kind, region):
+    """Predictive scaled-FP16 strips (the inner ring part of theta and the
+    gradients kept in FP32): the image against the FP64 oracle with exact
+    strips, at the image bar."""
+    _, errs = _small_errors(oracle, engine, "faithful", abi.STRIP_FP16)
+    err = errs[(kind, region)]
+    _, e32 = _small_errors(oracle, engine, "faithful")
+    print(f"[fp16 strips {engine}] {kind} {region}: {err:.2e} (FP32 strips {e32[(kind, region)]:.2e})", flush=True)
+    assert err < IMAGE_BAR, err
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_fp16_strips_cost_nothing_on_the_receiver_rows(oracle, engine):
+    """A shot 16 cells from the left edge of a 320 x 96 grid, 2600 steps: the
+    geometry where residual-coded FP16 strips made the rows next to the
+    receiver row 3-15x worse than FP32 strips (and, at 2048 x 1024, flipped
+    the sign of PS cells on row 0).  With the inner ring part of theta and the
+    four gradients kept in FP32, the FP16-strip image error is within 10 % of
+    the FP32-strip image's below row 8, and within 2x on rows 0-7 (measured:
+    plain 1.01x / 1.03x, fused PP 1.04x, fused PS 1.52x on rows 0-7, where the
+    outer ring part of theta, still coded, reaches rows 0-3)."""
+    nx, nz, nt = 320, 96, 2600
     vp, vs, _ = vm.layered_random(nx, nz, seed=3)
-    p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, strip_precision=abi.STRIP_FP16,
-                        engine=engine)
-    shot = dict(sx=80, sz=0, rec_z=0, fm=22.0, mute=1)
-    with Engine(p, vp, vs) as e:
-        img = e.rtm_shot(shot)
-    emu = oracle.rtm_shot(p, vp, vs, shot, 64)  # FP64 arithmetic, same FP16 codec
-    p.strip_precision = abi.STRIP_FP32
-    ref = oracle.rtm_shot(p, vp, vs, shot, 64)
-    gpu = rel_l2(img["img_pp"][:, 8:], ref["img_pp"][:, 8:])
-    cod = rel_l2(emu["img_pp"][:, 8:], ref["img_pp"][:, 8:])
-    print(f"[fp16 strips, engine {engine}] below row 8: GPU {gpu:.2e}, FP64+codec {cod:.2e}", flush=True)
-    assert gpu < 1e-4
+    shot = dict(sx=16, sz=0, rec_z=0, fm=20.0, mute=1)
+    p = abi.make_params(nx, nz, 30, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, engine=ENGINES[engine])
+    ref = _memo("edge_ref", lambda: oracle.rtm_shot(p, vp, vs, shot, 64))
+    errs = {}
+    for strips in (abi.STRIP_FP32, abi.STRIP_FP16):
+        p.strip_precision = strips
+        print(f"[case] rtm 320x96 nt=2600 sx=16 {engine} strips={strips}", flush=True)
+        with Engine(p, vp, vs) as e:
+            errs[strips] = errors_full(e.rtm_shot(shot), ref, shot["sx"])
+    for k in ("img_pp", "img_ps"):
+        for r in ("top", "below8"):
+            a, b = errs[abi.STRIP_FP16][(k, r)], errs[abi.STRIP_FP32][(k, r)]
+            print(f"[edge shot {engine}] {k} {r}: FP16 strips {a:.2e}, FP32 strips {b:.2e}", flush=True)
+            assert a < (2.0 if r == "top" else 1.1) * b, (k, r, a, b)

@@ -1,46 +1,52 @@
 """GPU split-PML first-order engine (OUC_ENGINE_SPLIT_PML) through the C ABI
 against its FP64 oracle, on the 200 x 200 two-layer forward case (2000 steps,
-20-cell damping at R = 1e-4).  Totals meet the 1e-5 field bar.  The P/S split
-parts of this formulation are ill-conditioned in FP32 (the C - E correction is a
-difference of large integrated fields, and the S part is mostly made in the
-damping pad): two FP32 variants of the oracle that differ only in dividing by dx
-or multiplying by 1/dx differ by up to 2x in them.  They are printed beside the
-FP32 oracle's own error and held within 10x of it (measured 4.2-4.6x: vpx 9.7e-4,
-vpz 1.5e-3, vsx 2.8e-2, vsz 6.1e-2 against 2.3e-4, 4.1e-4, 6.6e-3, 1.6e-2)."""
+20-cell damping at R = 1e-4).  Each seismogram component is held to the 1e-5
+field bar.  Totals meet it.  The P/S split parts of this formulation are
+ill-conditioned in FP32 (the C - E correction is a difference of large
+integrated fields, and the S part is mostly made in the damping pad): two FP32
+variants of the oracle that differ only in dividing by dx or multiplying by
+1/dx differ by up to 2x in them; they are marked xfail(strict=True) with the
+engine's measured error, and the FP32 oracle's own error is printed beside it."""
 import numpy as np
 import pytest
 
 from conftest import rel_l2
+from image_metrics import FIELD_BAR, bar_param
 from ouc_amd import abi, velocity_models as vm
 from ouc_amd.engine import Engine, OucError
 
 pytestmark = pytest.mark.gpu
 
 COMP = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
+SPLIT_MISSES = {"vpx": 9.7e-4, "vpz": 1.5e-3, "vsx": 2.8e-2, "vsz": 6.1e-2}
+_run = {}
 
 
-def test_splitpml_forward_two_layer_200(oracle):
-    p, vp, vs, shot = vm.cfg_forward_200()
-    ref = oracle.forward_splitpml(p, vp, vs, shot, 64)
-    r32 = oracle.forward_splitpml(p, vp, vs, shot, 32)
-    p.engine = abi.ENGINE_SPLIT_PML
-    print(f"[case] split-PML forward 200x200 N=5 nt={p.nt}", flush=True)
-    with Engine(p, vp, vs) as e:
-        seis = e.forward_shot(shot)
-        ms = e.timings()["total_ms"]
-        with pytest.raises(OucError) as err:
-            e.rtm_shot(shot)
-        assert err.value.status == abi.OUC_UNSUPPORTED
-    errs = {c: rel_l2(seis[q], ref[q]) for q, c in enumerate(COMP)}
-    e32 = {c: rel_l2(r32[q], ref[q]) for q, c in enumerate(COMP)}
-    cells = (p.nx + 2 * p.npml - 10) * (p.nz + 2 * p.npml - 10) * p.nt
-    print("[split-PML 200] rel-L2 vs FP64:", " ".join(f"{c}={v:.2e}" for c, v in errs.items()),
-          "| FP32 oracle:", " ".join(f"{c}={v:.2e}" for c, v in e32.items()),
-          f"| {ms:.1f} ms, {cells / ms / 1e6:.2f} Gcell/s", flush=True)
-    for c in ("vx", "vz"):
-        assert errs[c] < 1e-5, (c, errs[c])
-    for c in ("vpx", "vpz", "vsx", "vsz"):
-        assert errs[c] < 10 * max(e32[c], 1e-6), (c, errs[c], e32[c])
+def _splitpml_200(oracle):
+    if not _run:
+        p, vp, vs, shot = vm.cfg_forward_200()
+        _run["ref"] = oracle.forward_splitpml(p, vp, vs, shot, 64)
+        _run["r32"] = oracle.forward_splitpml(p, vp, vs, shot, 32)
+        p.engine = abi.ENGINE_SPLIT_PML
+        print(f"[case] split-PML forward 200x200 N=5 nt={p.nt}", flush=True)
+        with Engine(p, vp, vs) as e:
+            _run["seis"] = e.forward_shot(shot)
+            ms = e.timings()["total_ms"]
+            with pytest.raises(OucError) as err:
+                e.rtm_shot(shot)
+            assert err.value.status == abi.OUC_UNSUPPORTED
+        cells = (p.nx + 2 * p.npml - 10) * (p.nz + 2 * p.npml - 10) * p.nt
+        print(f"[split-PML 200] {ms:.1f} ms, {cells / ms / 1e6:.2f} Gcell/s", flush=True)
+    return _run
+
+
+@pytest.mark.parametrize("comp", [bar_param(c, SPLIT_MISSES, FIELD_BAR, c, id=c) for c in COMP])
+def test_splitpml_forward_two_layer_200(oracle, comp):
+    r = _splitpml_200(oracle)
+    q = COMP.index(comp)
+    err = rel_l2(r["seis"][q], r["ref"][q])
+    print(f"[split-PML 200] {comp}: {err:.2e} (FP32 oracle {rel_l2(r['r32'][q], r['ref'][q]):.2e})", flush=True)
+    assert err < FIELD_BAR, err
 
 
 def test_splitpml_twelfth_order_coefficients(oracle):
