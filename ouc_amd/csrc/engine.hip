@@ -35,6 +35,16 @@ void DevBuf::alloc(size_t n) {
     bytes = n;
 }
 
+void copy_h2d(hipStream_t s, void *dst, const void *src, size_t bytes) {
+    OUC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    OUC_HIP(hipStreamSynchronize(s));
+}
+
+void copy_d2h(hipStream_t s, void *dst, const void *src, size_t bytes) {
+    OUC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    OUC_HIP(hipStreamSynchronize(s));
+}
+
 void DevBuf::release() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
@@ -414,10 +424,10 @@ ouc_status ouc_create(const ouc_params *p, const ouc_model *m, int32_t device, o
         c->vp_pad = vp;
         const size_t pn = g.pitched() * sizeof(float);
         c->p2.alloc(pn); c->q2.alloc(pn); c->ax.alloc(pn); c->az.alloc(pn);
-        OUC_HIP(hipMemcpy(c->p2.ptr, p2.data(), pn, hipMemcpyHostToDevice));
-        OUC_HIP(hipMemcpy(c->q2.ptr, q2.data(), pn, hipMemcpyHostToDevice));
-        OUC_HIP(hipMemcpy(c->ax.ptr, ax.data(), pn, hipMemcpyHostToDevice));
-        OUC_HIP(hipMemcpy(c->az.ptr, az.data(), pn, hipMemcpyHostToDevice));
+        copy_h2d(c->stream, c->p2.ptr, p2.data(), pn);
+        copy_h2d(c->stream, c->q2.ptr, q2.data(), pn);
+        copy_h2d(c->stream, c->ax.ptr, ax.data(), pn);
+        copy_h2d(c->stream, c->az.ptr, az.data(), pn);
         c->F.alloc(g);
         c->R.alloc(g);
         for (auto &lv : c->vp3)

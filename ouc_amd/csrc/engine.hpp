@@ -25,6 +25,13 @@ void hip_check(hipError_t e, const char *what);
 ouc_status ouc_fail_global(ouc_status code, const std::string &msg);
 #define OUC_HIP(x) ::ouc::hip_check((x), #x)
 
+// Host <-> device copies on a context's own (non-blocking) stream, waited
+// for.  The library issues nothing on the legacy null stream: a legacy-stream
+// copy made while another thread's context captures a time loop into a graph
+// would be ordered against that capture.
+void copy_h2d(hipStream_t s, void *dst, const void *src, size_t bytes);
+void copy_d2h(hipStream_t s, void *dst, const void *src, size_t bytes);
+
 // Owning device allocation.
 struct DevBuf {
     void *ptr = nullptr;

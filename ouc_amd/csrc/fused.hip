@@ -89,6 +89,7 @@ struct FusedState {
     DevBuf img;            // [4][nx*nz] scratch for one shot's images
     // learned boundary
     DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
+    DevBuf lbd; // [cap][nlbcells]: the network ring's new increments of a step
     int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
     bool lb_have = false;  // composed filter present (else the ring is held at zero)
     int csx = kCoreSX, pf = kCorePF; // core tile
@@ -239,14 +240,14 @@ std::vector<int2> edge_tiles(const Grid &g, int depth) {
     return out;
 }
 
-void build_learned(FusedState *st, const Grid &g) {
+void build_learned(FusedState *st, const Grid &g, hipStream_t s) {
     std::vector<int2> tp = edge_tiles(g, g.L), tm = edge_tiles(g, 2 * g.L);
     st->ntiles_p = (int)tp.size();
     st->ntiles_m = (int)tm.size();
     st->tiles_p.alloc(tp.size() * sizeof(int2));
     st->tiles_m.alloc(tm.size() * sizeof(int2));
-    OUC_HIP(hipMemcpy(st->tiles_p.ptr, tp.data(), tp.size() * sizeof(int2), hipMemcpyHostToDevice));
-    OUC_HIP(hipMemcpy(st->tiles_m.ptr, tm.data(), tm.size() * sizeof(int2), hipMemcpyHostToDevice));
+    copy_h2d(s, st->tiles_p.ptr, tp.data(), tp.size() * sizeof(int2));
+    copy_h2d(s, st->tiles_m.ptr, tm.data(), tm.size() * sizeof(int2));
     // the network ring: velocity cells of the pad that the low-order scheme
     // cannot update and that are not the always-zero ghost line
     std::vector<LbCell> ring;
@@ -260,7 +261,7 @@ void build_learned(FusedState *st, const Grid &g) {
         }
     st->nlbcells = (int)ring.size();
     st->lbcells.alloc(ring.size() * sizeof(LbCell));
-    OUC_HIP(hipMemcpy(st->lbcells.ptr, ring.data(), ring.size() * sizeof(LbCell), hipMemcpyHostToDevice));
+    copy_h2d(s, st->lbcells.ptr, ring.data(), ring.size() * sizeof(LbCell));
 }
 
 size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
@@ -359,8 +360,8 @@ void ensure(ouc_ctx *c, int shots) {
     }
     st->tiles_s.alloc(ts.size() * sizeof(int2));
     st->tiles_v.alloc(tv.size() * sizeof(int2));
-    OUC_HIP(hipMemcpy(st->tiles_s.ptr, ts.data(), ts.size() * sizeof(int2), hipMemcpyHostToDevice));
-    OUC_HIP(hipMemcpy(st->tiles_v.ptr, tv.data(), tv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    copy_h2d(c->stream, st->tiles_s.ptr, ts.data(), ts.size() * sizeof(int2));
+    copy_h2d(c->stream, st->tiles_v.ptr, tv.data(), tv.size() * sizeof(int2));
     // reconstruction band: stress on the ring frame minus [L+3N, Xn-L-3N)^2,
     // velocity on the interior minus the recon core [L+2N, Xn-L-2N)^2
     const int L = g.L, N = g.N;
@@ -385,10 +386,13 @@ void ensure(ouc_ctx *c, int shots) {
     st->ntiles_rv = (int)trv.size();
     st->tiles_rs.alloc(trs.size() * sizeof(int2));
     st->tiles_rv.alloc(trv.size() * sizeof(int2));
-    OUC_HIP(hipMemcpy(st->tiles_rs.ptr, trs.data(), trs.size() * sizeof(int2), hipMemcpyHostToDevice));
-    OUC_HIP(hipMemcpy(st->tiles_rv.ptr, trv.data(), trv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    copy_h2d(c->stream, st->tiles_rs.ptr, trs.data(), trs.size() * sizeof(int2));
+    copy_h2d(c->stream, st->tiles_rv.ptr, trv.data(), trv.size() * sizeof(int2));
     st->mon.alloc((size_t)2 * shots * 2 * sizeof(unsigned int)); // [pass][shot][pad, band]
-    if (g.learned) build_learned(st, g);
+    if (g.learned) {
+        build_learned(st, g, c->stream);
+        st->lbd.alloc((size_t)shots * st->nlbcells * sizeof(float));
+    }
 }
 
 KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
@@ -401,6 +405,7 @@ KStrips base_strips(const FusedState *st, const ouc_ctx *c) {
     s.err = st->ef.f();
     s.dec = st->decb.f();
     s.nouter = c->ring.nouter;
+    s.t0 = st->t0.as<int>();
     s.last_v = st->nslots - 2;  // velocity totals V^{t+1} are saved for t < nt-1
     s.last_th = st->nslots - 1; // the stress-like fields for every t
     s.ncells = c->ring.ncells;
@@ -505,8 +510,10 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
     if (f.lpad) {
         // the ring reads the pad's increments before the pad update advances them
         const float sc = f.dt2 * f.idx;
-        k_lb_ring<<<dim3((st->nlbcells + 255) / 256, 1, B), 256, 0, s>>>(
-            f, w, km, st->lbcells.as<LbCell>(), st->nlbcells, st->lb_have ? st->lbw.f() : nullptr, a.rb, sc);
+        const dim3 rg((st->nlbcells + 255) / 256, 1, B);
+        k_lb_ring<<<rg, 256, 0, s>>>(f, w, km, st->lbcells.as<LbCell>(), st->nlbcells,
+                                     st->lb_have ? st->lbw.f() : nullptr, a.rb, sc, st->lbd.f());
+        if (st->lb_have) k_lb_apply<<<rg, 256, 0, s>>>(f, w, st->lbcells.as<LbCell>(), st->nlbcells, st->lbd.f(), a.rb);
         k_lpad_velocity<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a.rb);
         if (a.t % kMonStride == 0)
             k_lb_monitor<<<dim3(st->ntiles_m, 1, B), dim3(64, 4), 0, s>>>(f, w, st->tiles_m.as<int2>(), 1 - a.rb, mon);
@@ -559,6 +566,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     for (int b = 0; b < B; ++b) {
         const ouc_shot &sh = shots[b];
         const int nts = sh.nt > 0 ? sh.nt : nt;
+        if (nts < 2 || nts > nt) throw Error(OUC_INVALID_ARG, "shot nt out of range");
         start[b] = nt - nts; // a shorter shot starts late and ends with the others
         pos[2 * b] = sh.sx + g.L;
         pos[2 * b + 1] = sh.sz + g.L;
@@ -1186,14 +1194,14 @@ void fused_set_learned(ouc_ctx *c, const float *weights, int n) {
                                     W3[(co * 16 + h2) * 9 + y3 * 3 + x3] * A[((size_t)h2 * 5 + ci) * 25 + y * 5 + x];
     std::vector<float> kf(K.begin(), K.end());
     if (!st->lbw.ptr) st->lbw.alloc(kLbTaps * sizeof(float));
-    OUC_HIP(hipMemcpy(st->lbw.ptr, kf.data(), kLbTaps * sizeof(float), hipMemcpyHostToDevice));
+    copy_h2d(c->stream, st->lbw.ptr, kf.data(), kLbTaps * sizeof(float));
     st->lb_have = true;
 }
 
 void fused_get_learned_filter(ouc_ctx *c, float *taps) {
     FusedState *st = fs(c);
     if (!st->lb_have) throw Error(OUC_INVALID_ARG, "no network weights set");
-    OUC_HIP(hipMemcpy(taps, st->lbw.ptr, kLbTaps * sizeof(float), hipMemcpyDeviceToHost));
+    copy_d2h(c->stream, taps, st->lbw.ptr, kLbTaps * sizeof(float));
 }
 
 // RTM of one shot with snapshots of the reconstructed source V^t over a

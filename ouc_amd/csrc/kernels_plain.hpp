@@ -394,12 +394,14 @@ struct KStrips {
     __half *f16;         // [slot][strip16_offset(.., RS_NF)] residuals (FP16 strips), field f at strip16_offset(f)
     float *in32;         // [slot][kNInner32][ncells - nouter] inner ring part of the FP32-kept fields (FP16 strips)
     int8_t *e8;          // [slot][RS_NF][nchunks] chunk exponents
-    float *key;          // [slot / kStripKey][RS_NF][ncells] keyframes
+    float *key;          // [(slot - key0) / kStripKey][RS_NF][ncells] keyframes
     float *err;          // [2][RS_NF][ncells] encoder state: previous decoded value, fed-back error
     float *dec;          // [2][RS_NF][ncells] decoded slots, by slot parity
     long ncells, nchunks, nouter;
     int fp16;
     int last_v, last_th; // last slot stored for the two velocity fields / the six stress-like fields
+    const int *t0;       // batches: [shot] first step of each shot, or NULL (every shot starts at 0)
+    int key0;            // this shot's first slot: keyframes are every kStripKey-th slot from it
 };
 
 // Element strides between the strip stores of consecutive shots of a batch.
@@ -412,6 +414,7 @@ struct KStripStride {
 };
 
 __device__ __forceinline__ KStrips strips_of_shot(KStrips s, const KStripStride &d, long z) {
+    s.key0 = s.t0 ? s.t0[z] : 0;
     if (s.f32) s.f32 += z * d.strip;
     if (s.f16) s.f16 += z * d.strip;
     if (s.in32) s.in32 += z * d.in32;
@@ -436,8 +439,15 @@ __device__ __forceinline__ int scale_exponent(float mx) {
     return e < -127 ? -127 : (e > 127 ? 127 : e);
 }
 
+// Keyframes fall on every kStripKey-th slot counted from the shot's own first
+// slot, so a shot that starts late in a batch codes its strips exactly as a
+// run of its own step count does, and on each field's last slot.
 __device__ __forceinline__ bool strip_keyframe(const KStrips &s, int slot, int f) {
-    return slot % kStripKey == kStripKey - 1 || slot == (f < RS_TH ? s.last_v : s.last_th);
+    const int rel = slot - s.key0;
+    return (rel >= 0 && rel % kStripKey == kStripKey - 1) || slot == (f < RS_TH ? s.last_v : s.last_th);
+}
+__device__ __forceinline__ long strip_key_index(const KStrips &s, int slot, int f) {
+    return ((long)((slot - s.key0) / kStripKey) * RS_NF + f) * s.ncells;
 }
 
 // Store `v` of ring cell q (one lane per cell, 64 consecutive cells per wave;
@@ -466,7 +476,7 @@ __device__ __forceinline__ void strip_put(const KStrips &s, int slot, int f, lon
         p1 = isfinite(p1) ? p1 : 0.f;
         *pp = p1;
         *rp = isfinite(r) ? r : 0.f;
-        if (strip_keyframe(s, slot, f)) s.key[((long)(slot / kStripKey) * RS_NF + f) * s.ncells + q] = p1;
+        if (strip_keyframe(s, slot, f)) s.key[strip_key_index(s, slot, f) + q] = p1;
         if ((q & 63) == 0) s.e8[((long)slot * RS_NF + f) * s.nchunks + (q >> 6)] = (int8_t)e;
     }
 }
@@ -496,7 +506,7 @@ static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slo
         if (inner32_field(f) && q >= s.nouter) {
             v = s.in32[((long)slot * kNInner32 + inner32_rank(f)) * (s.ncells - s.nouter) + (q - s.nouter)];
         } else if (strip_keyframe(s, slot, f)) {
-            v = key[((long)(slot / kStripKey) * RS_NF + f) * s.ncells + q];
+            v = key[strip_key_index(s, slot, f) + q];
         } else {
             const long b1 = (long)(slot + 1) * strip16_offset(s.ncells, s.nouter, RS_NF) +
                             strip16_offset(s.ncells, s.nouter, f);

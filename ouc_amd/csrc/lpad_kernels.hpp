@@ -149,10 +149,13 @@ __device__ __forceinline__ int lb_map(const FGrid &g, int side, int cc, int a, i
 // Network on the outer ring: for each ring cell, the side with the smallest
 // canonical depth (ties: x sides first, low before high), the 7x7x5 canonical
 // window, the composed filter, and the correction added to the increment:
-// D += delta, V_new = V + D.  With wts == NULL the ring is held at zero (the
-// pad without the network).
+// D += delta, V_new = V + D.  Every window reads the increments before any
+// ring cell advances them (a window holds other ring cells), so the new
+// increments go to dnew[shot][ncells] and k_lb_apply stores them.  With
+// wts == NULL the ring is held at zero here (the pad without the network).
 __global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, const LbCell *__restrict__ cells,
-                                                 int ncells, const float *__restrict__ wts, int rb, float sc) {
+                                                 int ncells, const float *__restrict__ wts, int rb, float sc,
+                                                 float *__restrict__ dnew) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ncells) return;
     const LbCell cl = cells[e];
@@ -195,9 +198,19 @@ __global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, con
     float sg_out = 1.f;
     if (side == 1) sg_out = kLbXS[f];
     if (side == 3) sg_out = kLbZS[f];
-    const float d = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
-    w.DV[f][k] = d;
-    w.V[wb][f][k] = w.V[rb][f][k] + d;
+    dnew[(long)blockIdx.z * ncells + e] = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
+}
+
+// The ring's new increments from k_lb_ring: D = dnew, V_new = V + D.
+__global__ __launch_bounds__(256) void k_lb_apply(FGrid g, FWave w, const LbCell *__restrict__ cells, int ncells,
+                                                  const float *__restrict__ dnew, int rb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ncells) return;
+    const LbCell cl = cells[e];
+    const long k = (long)blockIdx.z * g.plane + (long)cl.i * g.P + cl.j;
+    const float d = dnew[(long)blockIdx.z * ncells + e];
+    w.DV[cl.f][k] = d;
+    w.V[1 - rb][cl.f][k] = w.V[rb][cl.f][k] + d;
 }
 
 // Largest |V| of each shot over the pad (mon[2 shot]) and over the band of the

@@ -172,6 +172,24 @@ def test_trained_network_beats_the_band_without_it(LB, weights):
         assert r["learned"] < r["cpml_band"] - 8.0
 
 
+def test_trained_network_at_the_training_size(LB, weights):
+    """The learned-boundary configuration at its own size: 512 x 512 interiors
+    (1,836 steps) inside 1536 x 1536 extended runs with a 40-cell CPML, two
+    random layered models the network never saw (vp 1500-4500 m/s, Ricker
+    10-40 Hz).  Reflected energy against the extended run: the trained network
+    on the 8-cell band against the band without it, the reference's CPML on
+    the same 8 cells, and the 40-cell CPML it replaces (the fallback)."""
+    for k in range(2):
+        c = LB.Case(512, 9100 + k)
+        print(f"[case] learned boundary 512x512 nt={c.nt} seed {9100 + k}", flush=True)
+        r = LB.reflection_db(c, weights, variants=("learned", "no_network", "cpml_band", "cpml_40"), stride=16)
+        print(f"[512 case {k}] vp {c.vp_ext.min():.0f}-{c.vp_ext.max():.0f} m/s, fm {c.fm:.1f} Hz: "
+              + ", ".join(f"{a} {b:.1f} dB" for a, b in r.items()), flush=True)
+        assert r["learned"] < -12.0
+        assert r["learned"] < r["no_network"] - 8.0
+        assert r["learned"] < r["cpml_band"] - 8.0
+
+
 def test_long_runs_stay_stable(LB, weights):
     """8000 steps (the longest serving shots) on the model whose undamped run
     grew after ~4000: the default pad damping keeps the pad below the band
