@@ -223,11 +223,14 @@ struct Wave {
     }
 };
 
-/* Planted-mistake switch for the CPML pins (tests only, default 0 = none).
+/* Planted-mistake switch for the pins (tests only, default 0 = none).
  * Value u in 1..12 swaps the damping profile (ax <-> az) of one memory-variable
  * update: 1..4 the stress pass's F_xx, F_zz, F_xz, F_zx; 5..12 the velocity
  * pass's O_C, O_E, O_B, O_A, O_Tx, O_Tz, O_Wz, O_Wx.  A test shows that each
- * such mistake breaks the absorbing boundary. */
+ * such mistake breaks the absorbing boundary.  In the split-PML
+ * reconstruction, 13 restores the ring velocities one slot late and 14 drops
+ * the source sample from the reversed stress pass; each breaks its exact
+ * reversibility. */
 int g_mutation = 0;
 
 /* Damping profile of memory-variable update u (1-based, as g_mutation). */
@@ -702,11 +705,164 @@ void gradfun_step(const Setup<R> &S, const R *ux, const R *uz, const R *th, cons
  * `if (j = Z_receive)` assigns rather than compares, so every thread writes
  * the receiver row). */
 template <class R>
-int run_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, double *seis) {
+void laplace_interior(int nx, int nz, R dh, const R *img, R *lap);
+
+/* One wavefield of the first-order engine: totals, P/S velocities, the
+ * stress-like fields and their x / z split parts. */
+template <class R>
+struct SplitWave {
+    std::vector<R> vx, vz, vpx, vpz, vsx, vsz, th, om, thx, thz, omx, omz, uxx, uxz, uzx, uzz;
+    void init(size_t n) {
+        for (auto *v : {&vx, &vz, &vpx, &vpz, &vsx, &vsz, &th, &om, &thx, &thz, &omx, &omz, &uxx, &uxz, &uzx, &uzz})
+            v->assign(n, (R)0);
+    }
+};
+
+/* `stress` (P-S-seperating.cu:487-555; as stress_backward_propagation,
+ * P-S-seperating.cu:815-886, for the receiver run) on [N, Xn-N)^2, adding s
+ * at (si, sj) to theta. */
+template <class R>
+void sp_stress(const Setup<R> &S, const R *c, SplitWave<R> &W, int si, int sj, R s) {
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn;
+    const R dt = S.dt, half = (R)0.5, one = (R)1;
+#pragma omp parallel for schedule(static)
+    for (int i = N; i < g.Xn - N; ++i)
+        for (int j = N; j < Zn - N; ++j) {
+            size_t k = (size_t)i * Zn + j;
+            R dvxdx = dmx(W.vx.data(), k, Zn, c, N, S.dx), dvxdz = dpz(W.vx.data(), k, c, N, S.dz);
+            R dvzdz = dmz(W.vz.data(), k, c, N, S.dz), dvzdx = dpx(W.vz.data(), k, Zn, c, N, S.dx);
+            R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
+            R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
+            R p2 = S.vp[k] * S.vp[k], q2 = S.vs[k] * S.vs[k];
+            W.thx[k] = ax1 * (dt * p2 * dvxdx + ax2 * W.thx[k]);
+            W.thz[k] = az1 * (dt * p2 * dvzdz + az2 * W.thz[k]);
+            W.omx[k] = ax1 * (-dt * q2 * dvzdx + ax2 * W.omx[k]);
+            W.omz[k] = az1 * (dt * q2 * dvxdz + az2 * W.omz[k]);
+            W.uxx[k] = ax1 * (dt * q2 * dvxdx + ax2 * W.uxx[k]);
+            W.uxz[k] = az1 * (dt * q2 * dvxdz + az2 * W.uxz[k]);
+            W.uzx[k] = ax1 * (dt * q2 * dvzdx + ax2 * W.uzx[k]);
+            W.uzz[k] = az1 * (dt * q2 * dvzdz + az2 * W.uzz[k]);
+            W.th[k] = W.thx[k] + W.thz[k] + ((i == si && j == sj) ? s : (R)0);
+            W.om[k] = W.omx[k] + W.omz[k];
+        }
+}
+
+/* `velocity` (P-S-seperating.cu:390-485; as velocity_backward_propagation,
+ * P-S-seperating.cu:728-813, for the receiver run) on [N, Xn-N)^2. */
+template <class R>
+void sp_velocity(const Setup<R> &S, const R *c, SplitWave<R> &W) {
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn;
+    const R dt = S.dt, half = (R)0.5, one = (R)1;
+#pragma omp parallel for schedule(static)
+    for (int i = N; i < g.Xn - N; ++i)
+        for (int j = N; j < Zn - N; ++j) {
+            size_t k = (size_t)i * Zn + j;
+            R dthdx = dpx(W.th.data(), k, Zn, c, N, S.dx), dthdz = dpz(W.th.data(), k, c, N, S.dz);
+            R domdx = dmx(W.om.data(), k, Zn, c, N, S.dx), domdz = dmz(W.om.data(), k, c, N, S.dz);
+            R duxxdz = dpz(W.uxx.data(), k, c, N, S.dz), duxzdx = dmx(W.uxz.data(), k, Zn, c, N, S.dx);
+            R duzxdz = dmz(W.uzx.data(), k, c, N, S.dz), duzzdx = dpx(W.uzz.data(), k, Zn, c, N, S.dx);
+            R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
+            R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
+            W.vpx[k] = ax1 * (ax2 * W.vpx[k] + dt * (dthdx + duzxdz - duzzdx));
+            W.vpz[k] = az1 * (az2 * W.vpz[k] + dt * (dthdz + duxzdx - duxxdz));
+            W.vsx[k] = ax1 * (ax2 * W.vsx[k] + dt * (domdz + duzxdz - duzzdx));
+            W.vsz[k] = az1 * (az2 * W.vsz[k] + dt * (-domdx + duxzdx - duxxdz));
+            W.vx[k] = W.vpx[k] + W.vsx[k];
+            W.vz[k] = W.vpz[k] + W.vsz[k];
+        }
+}
+
+/* Reverse of the interior velocity pass, velocity_wavefield_reconstruction
+ * (P-S-seperating.cu:1066-1167): on [L, Xn-L)^2, where the damping is zero,
+ * V^t = V^{t+1} - dt f(stress^{t+1}). */
+template <class R>
+void sp_recon_velocity(const Setup<R> &S, const R *c, SplitWave<R> &W) {
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn, L = g.L;
+    const R dt = S.dt;
+#pragma omp parallel for schedule(static)
+    for (int i = L; i < g.Xn - L; ++i)
+        for (int j = L; j < Zn - L; ++j) {
+            size_t k = (size_t)i * Zn + j;
+            R dthdx = dpx(W.th.data(), k, Zn, c, N, S.dx), dthdz = dpz(W.th.data(), k, c, N, S.dz);
+            R domdx = dmx(W.om.data(), k, Zn, c, N, S.dx), domdz = dmz(W.om.data(), k, c, N, S.dz);
+            R duxxdz = dpz(W.uxx.data(), k, c, N, S.dz), duxzdx = dmx(W.uxz.data(), k, Zn, c, N, S.dx);
+            R duzxdz = dmz(W.uzx.data(), k, c, N, S.dz), duzzdx = dpx(W.uzz.data(), k, Zn, c, N, S.dx);
+            W.vpx[k] = W.vpx[k] - dt * (dthdx + duzxdz - duzzdx);
+            W.vpz[k] = W.vpz[k] - dt * (dthdz + duxzdx - duxxdz);
+            W.vsx[k] = W.vsx[k] - dt * (domdz + duzxdz - duzzdx);
+            W.vsz[k] = W.vsz[k] - dt * (-domdx + duxzdx - duxxdz);
+            W.vx[k] = W.vpx[k] + W.vsx[k];
+            W.vz[k] = W.vpz[k] + W.vsz[k];
+        }
+}
+
+/* Reverse of the interior stress pass, stress_wavefield_reconstruction
+ * (P-S-seperating.cu:1168-1229): stress^t = stress^{t+1} - dt g(V^t) on
+ * [L, Xn-L)^2; theta^t carries the source sample s of forward step t-1 at
+ * (si, sj), as the forward theta^t does. */
+template <class R>
+void sp_recon_stress(const Setup<R> &S, const R *c, SplitWave<R> &W, int si, int sj, R s) {
+    const Geo &g = S.g;
+    const int N = g.N, Zn = g.Zn, L = g.L;
+    const R dt = S.dt;
+#pragma omp parallel for schedule(static)
+    for (int i = L; i < g.Xn - L; ++i)
+        for (int j = L; j < Zn - L; ++j) {
+            size_t k = (size_t)i * Zn + j;
+            R dvxdx = dmx(W.vx.data(), k, Zn, c, N, S.dx), dvxdz = dpz(W.vx.data(), k, c, N, S.dz);
+            R dvzdz = dmz(W.vz.data(), k, c, N, S.dz), dvzdx = dpx(W.vz.data(), k, Zn, c, N, S.dx);
+            R p2 = S.vp[k] * S.vp[k], q2 = S.vs[k] * S.vs[k];
+            W.thx[k] = W.thx[k] - dt * p2 * dvxdx;
+            W.thz[k] = W.thz[k] - dt * p2 * dvzdz;
+            W.omx[k] = W.omx[k] + dt * q2 * dvzdx;
+            W.omz[k] = W.omz[k] - dt * q2 * dvxdz;
+            W.uxx[k] = W.uxx[k] - dt * q2 * dvxdx;
+            W.uxz[k] = W.uxz[k] - dt * q2 * dvxdz;
+            W.uzx[k] = W.uzx[k] - dt * q2 * dvzdx;
+            W.uzz[k] = W.uzz[k] - dt * q2 * dvzdz;
+            W.th[k] = W.thx[k] + W.thz[k] + ((i == si && j == sj) ? s : (R)0);
+            W.om[k] = W.omx[k] + W.omz[k];
+        }
+}
+
+/* First-order velocity-stress engine with split PML: `stress`
+ * (P-S-seperating.cu:487-555) then `velocity` (P-S-seperating.cu:390-485) per
+ * step, never launched by the reference's main().  The damping factors
+ * (1 / (1 + dt a / 2)) and (1 - dt a / 2) are evaluated in R (the reference
+ * mixes double literals into float expressions).  At N = 6 the coefficients are
+ * these kernels' five-digit literals.  Records: the six components at row Zr
+ * of every interior column after each velocity pass (the reference's
+ * `if (j = Z_receive)` assigns rather than compares, so every thread writes
+ * the receiver row).
+ *
+ * RTM (rtm = true) with this engine, which the reference declares but never
+ * drives, defined as the live engine's shot loop with aligned time levels:
+ *  - the forward run saves, after each step t, the ring (Ring) of vx, vz,
+ *    theta, omega and the four du fields (save_wavefiled,
+ *    P-S-seperating.cu:1230-1287), exactly;
+ *  - the records (vx, vz on row Zr, muted as the live engine mutes them) are
+ *    injected by replacement into the receiver run (load_record,
+ *    P-S-seperating.cu:1471-1483), which steps with the damped kernels;
+ *  - backward step t correlates the source P velocity of level t+1 with the
+ *    receiver's P and S velocities after its step t (corr_v,
+ *    P-S-seperating.cu:1426-1439), then reverses the source from level t+1 to
+ *    t: ring stress of level t+1 restored, the velocity pass reversed, ring
+ *    velocities of level t restored (zero at t = 0), the stress pass
+ *    reversed.  In exact arithmetic the reconstruction equals the forward run.
+ * recon_err (optional, [nt]) receives max |Vpx_recon^t - Vpx_fwd^t| over the
+ * interior divided by the largest |Vpx_fwd| (tiny grids: the forward levels
+ * are kept). */
+template <class R>
+int run_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, double *seis, bool rtm = false,
+                 double *img_pp = nullptr, double *img_ps = nullptr, double *lap_pp = nullptr,
+                 double *lap_ps = nullptr, double *recon_err = nullptr) {
     Setup<R> S;
     if (!make_setup(p, m, s, S)) return OUC_INVALID_ARG;
     const Geo &g = S.g;
-    const int N = g.N, Zn = g.Zn, nt = g.nt, nx = g.nx, L = g.L;
+    const int N = g.N, Zn = g.Zn, nt = g.nt, nx = g.nx, nz = g.nz, L = g.L;
     if (s->sx < 0 || s->sx >= nx || s->sz < 0 || s->sz >= g.nz || s->rec_z < 0 || s->rec_z >= g.nz)
         return OUC_INVALID_ARG;
     R c[8];
@@ -714,53 +870,119 @@ int run_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, dou
     for (int k = 0; k < N; ++k) c[k] = N == 6 ? (R)a6[k] : S.c[k];
     const int Sx = s->sx + L, Sz = s->sz + L, Zr = s->rec_z + L;
     const size_t n = (size_t)g.Xn * Zn;
-    std::vector<R> vx(n, 0), vz(n, 0), vpx(n, 0), vpz(n, 0), vsx(n, 0), vsz(n, 0), th(n, 0), om(n, 0);
-    std::vector<R> thx(n, 0), thz(n, 0), omx(n, 0), omz(n, 0), uxx(n, 0), uxz(n, 0), uzx(n, 0), uzz(n, 0);
-    const R dt = S.dt, half = (R)0.5, one = (R)1;
+    SplitWave<R> F;
+    F.init(n);
+    Ring ring;
+    ring.init(g);
+    std::vector<R> store;
+    if (rtm) store.assign((size_t)nt * RS_NF * ring.ncells, (R)0);
+    auto fields = [](SplitWave<R> &W) {
+        return std::vector<std::vector<R> *>{&W.vx, &W.vz, &W.th, &W.om, &W.uzx, &W.uzz, &W.uxx, &W.uxz};
+    };
+    std::vector<std::vector<R>> rec(2, std::vector<R>((size_t)nx * nt, (R)0));
+    std::vector<std::vector<R>> fwd_px;
+    if (rtm && recon_err) fwd_px.assign(nt + 1, std::vector<R>());
+    if (rtm && recon_err) fwd_px[0].assign(n, (R)0);
     for (int t = 0; t < nt; ++t) {
-#pragma omp parallel for schedule(static)
-        for (int i = N; i < g.Xn - N; ++i)
-            for (int j = N; j < Zn - N; ++j) {
-                size_t k = (size_t)i * Zn + j;
-                R dvxdx = dmx(vx.data(), k, Zn, c, N, S.dx), dvxdz = dpz(vx.data(), k, c, N, S.dz);
-                R dvzdz = dmz(vz.data(), k, c, N, S.dz), dvzdx = dpx(vz.data(), k, Zn, c, N, S.dx);
-                R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
-                R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
-                R p2 = S.vp[k] * S.vp[k], q2 = S.vs[k] * S.vs[k];
-                thx[k] = ax1 * (dt * p2 * dvxdx + ax2 * thx[k]);
-                thz[k] = az1 * (dt * p2 * dvzdz + az2 * thz[k]);
-                omx[k] = ax1 * (-dt * q2 * dvzdx + ax2 * omx[k]);
-                omz[k] = az1 * (dt * q2 * dvxdz + az2 * omz[k]);
-                uxx[k] = ax1 * (dt * q2 * dvxdx + ax2 * uxx[k]);
-                uxz[k] = az1 * (dt * q2 * dvxdz + az2 * uxz[k]);
-                uzx[k] = ax1 * (dt * q2 * dvzdx + ax2 * uzx[k]);
-                uzz[k] = az1 * (dt * q2 * dvzdz + az2 * uzz[k]);
-                th[k] = thx[k] + thz[k] + ((i == Sx && j == Sz) ? S.src[t] : (R)0);
-                om[k] = omx[k] + omz[k];
-            }
-#pragma omp parallel for schedule(static)
-        for (int i = N; i < g.Xn - N; ++i)
-            for (int j = N; j < Zn - N; ++j) {
-                size_t k = (size_t)i * Zn + j;
-                R dthdx = dpx(th.data(), k, Zn, c, N, S.dx), dthdz = dpz(th.data(), k, c, N, S.dz);
-                R domdx = dmx(om.data(), k, Zn, c, N, S.dx), domdz = dmz(om.data(), k, c, N, S.dz);
-                R duxxdz = dpz(uxx.data(), k, c, N, S.dz), duxzdx = dmx(uxz.data(), k, Zn, c, N, S.dx);
-                R duzxdz = dmz(uzx.data(), k, c, N, S.dz), duzzdx = dpx(uzz.data(), k, Zn, c, N, S.dx);
-                R hx = half * dt * S.ax[k], hz = half * dt * S.az[k];
-                R ax1 = one / (one + hx), az1 = one / (one + hz), ax2 = one - hx, az2 = one - hz;
-                vpx[k] = ax1 * (ax2 * vpx[k] + dt * (dthdx + duzxdz - duzzdx));
-                vpz[k] = az1 * (az2 * vpz[k] + dt * (dthdz + duxzdx - duxxdz));
-                vsx[k] = ax1 * (ax2 * vsx[k] + dt * (domdz + duzxdz - duzzdx));
-                vsz[k] = az1 * (az2 * vsz[k] + dt * (-domdx + duxzdx - duxxdz));
-                vx[k] = vpx[k] + vsx[k];
-                vz[k] = vpz[k] + vsz[k];
-            }
-        if (seis)
-            for (int ix = 0; ix < nx; ++ix) {
-                size_t k = (size_t)(ix + L) * Zn + Zr;
-                const R *f[6] = {vx.data(), vz.data(), vpx.data(), vpz.data(), vsx.data(), vsz.data()};
+        sp_stress(S, c, F, Sx, Sz, S.src[t]);
+        sp_velocity(S, c, F);
+        for (int ix = 0; ix < nx; ++ix) {
+            size_t k = (size_t)(ix + L) * Zn + Zr;
+            rec[0][(size_t)ix * nt + t] = F.vx[k];
+            rec[1][(size_t)ix * nt + t] = F.vz[k];
+            if (seis) {
+                const R *f[6] = {F.vx.data(), F.vz.data(), F.vpx.data(), F.vpz.data(), F.vsx.data(), F.vsz.data()};
                 for (int q = 0; q < 6; ++q) seis[((size_t)q * nx + ix) * nt + t] = (double)f[q][k];
             }
+        }
+        if (rtm) {
+            auto fs = fields(F);
+            for (int f = 0; f < RS_NF; ++f)
+                for (size_t r = 0; r < ring.ncells; ++r) {
+                    int i, j;
+                    ring.cell(r, i, j);
+                    store[((size_t)t * RS_NF + f) * ring.ncells + r] = (*fs[f])[(size_t)i * Zn + j];
+                }
+            if (recon_err) fwd_px[t + 1] = F.vpx;
+        }
+    }
+    if (!rtm) return OUC_OK;
+    if (s->mute) {
+        int t0 = ricker_t0(s->fm, p->dt);
+        float vsrc = (float)S.vp_d[(size_t)Sx * Zn + Zr];
+        for (int ix = 0; ix < nx; ++ix) {
+            double thr = mute_threshold(Sx, Sz, Zr, ix + L, t0, p->dt, p->dx, vsrc);
+            for (int t = 0; t < nt; ++t)
+                if (t < thr) { rec[0][(size_t)ix * nt + t] = 0; rec[1][(size_t)ix * nt + t] = 0; }
+        }
+    }
+    auto restore = [&](SplitWave<R> &W, int slot, int f0, int f1) {
+        auto fs = fields(W);
+        for (int f = f0; f < f1; ++f)
+            for (size_t r = 0; r < ring.ncells; ++r) {
+                int i, j;
+                ring.cell(r, i, j);
+                (*fs[f])[(size_t)i * Zn + j] = slot >= 0 ? store[((size_t)slot * RS_NF + f) * ring.ncells + r] : (R)0;
+            }
+    };
+    SplitWave<R> Rw;
+    Rw.init(n);
+    std::vector<R> num_pp(n, (R)0), num_ps(n, (R)0), den(n, (R)0);
+    double pmax = 0;
+    if (recon_err) {
+        for (int t = 1; t <= nt; ++t)
+            for (int i = L; i < g.Xn - L; ++i)
+                for (int j = L; j < Zn - L; ++j)
+                    pmax = std::max(pmax, std::fabs((double)fwd_px[t][(size_t)i * Zn + j]));
+        if (pmax == 0) pmax = 1;
+        recon_err[nt - 1 < 0 ? 0 : nt - 1] = 0;
+    }
+    for (int t = nt - 1; t >= 0; --t) {
+        for (int ix = 0; ix < nx; ++ix) { /* load_record */
+            size_t k = (size_t)(ix + L) * Zn + Zr;
+            Rw.vx[k] = rec[0][(size_t)ix * nt + t];
+            Rw.vz[k] = rec[1][(size_t)ix * nt + t];
+        }
+        sp_stress(S, c, Rw, -1, -1, (R)0);
+        sp_velocity(S, c, Rw);
+#pragma omp parallel for schedule(static)
+        for (int i = L; i < g.Xn - L; ++i)
+            for (int j = L; j < Zn - L; ++j) {
+                size_t k = (size_t)i * Zn + j;
+                R spx = F.vpx[k], spz = F.vpz[k];
+                num_pp[k] += spx * Rw.vpx[k] + spz * Rw.vpz[k];
+                den[k] += spx * spx + spz * spz;
+                num_ps[k] += spx * Rw.vsx[k] + spz * Rw.vsz[k];
+            }
+        /* the source from level t+1 to level t */
+        restore(F, t, RS_TH, RS_NF);
+        sp_recon_velocity(S, c, F);
+        restore(F, g_mutation == 13 ? t : t - 1, RS_VX, RS_VZ + 1);
+        sp_recon_stress(S, c, F, Sx, Sz, (t >= 1 && g_mutation != 14) ? S.src[t - 1] : (R)0);
+        if (recon_err) {
+            double e = 0;
+            for (int i = L; i < g.Xn - L; ++i)
+                for (int j = L; j < Zn - L; ++j) {
+                    size_t k = (size_t)i * Zn + j;
+                    e = std::max(e, std::fabs((double)F.vpx[k] - (double)fwd_px[t][k]));
+                }
+            recon_err[t] = e / pmax;
+        }
+    }
+    std::vector<R> cpp((size_t)nx * nz), cps((size_t)nx * nz), lpp((size_t)nx * nz), lps((size_t)nx * nz);
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) {
+            const size_t k = (size_t)(ix + L) * Zn + iz + L, oi = (size_t)ix * nz + iz;
+            cpp[oi] = num_pp[k] / den[k];
+            cps[oi] = num_ps[k] / den[k];
+        }
+    laplace_interior(nx, nz, S.dh, cpp.data(), lpp.data());
+    laplace_interior(nx, nz, S.dh, cps.data(), lps.data());
+    for (size_t oi = 0; oi < (size_t)nx * nz; ++oi) {
+        if (img_pp) img_pp[oi] = (double)cpp[oi];
+        if (img_ps) img_ps[oi] = (double)cps[oi];
+        if (lap_pp) lap_pp[oi] = (double)lpp[oi];
+        if (lap_ps) lap_ps[oi] = (double)lps[oi];
     }
     return OUC_OK;
 }
@@ -1197,7 +1419,7 @@ int oracle_laplace(int nx, int nz, double dh, const double *img, double *lap) {
 
 /* Planted-mistake switch of the CPML memory-variable updates (tests only). */
 int oracle_set_mutation(int u) {
-    if (u < 0 || u > 12) return OUC_INVALID_ARG;
+    if (u < 0 || u > 14) return OUC_INVALID_ARG;
     g_mutation = u;
     return OUC_OK;
 }
@@ -1206,6 +1428,16 @@ int oracle_set_mutation(int u) {
 int oracle_forward_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision, double *seis) {
     if (!p || !m || !s) return OUC_INVALID_ARG;
     return precision == 32 ? run_splitpml<float>(p, m, s, seis) : run_splitpml<double>(p, m, s, seis);
+}
+
+/* RTM with the first-order split-PML engine (run_splitpml): images, their
+ * Laplacians and, optionally, the reconstruction's error per level. */
+int oracle_rtm_splitpml(const ouc_params *p, const ouc_model *m, const ouc_shot *s, int precision, double *img_pp,
+                        double *img_ps, double *lap_pp, double *lap_ps, double *recon_err) {
+    if (!p || !m || !s) return OUC_INVALID_ARG;
+    return precision == 32
+               ? run_splitpml<float>(p, m, s, nullptr, true, img_pp, img_ps, lap_pp, lap_ps, recon_err)
+               : run_splitpml<double>(p, m, s, nullptr, true, img_pp, img_ps, lap_pp, lap_ps, recon_err);
 }
 
 /* FWI gradient of one shot: observed records [2][nx][nt]; grad_vp, grad_vs,

@@ -43,6 +43,7 @@ def lib():
         L.oracle_rtm_shot.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp, dp]
         L.oracle_recon_error.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
         L.oracle_forward_splitpml.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp]
+        L.oracle_rtm_splitpml.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp]
         L.oracle_fwi_gradient.argtypes = [P(abi.Params), P(abi.Model), P(abi.Shot), C.c_int, dp, dp, dp, dp, dp]
         L.oracle_gradfun.argtypes = [P(abi.Params), P(abi.Model)] + [dp] * 11
         L.oracle_set_threads.argtypes = [C.c_int]
@@ -186,6 +187,21 @@ def forward_splitpml(p, vp, vs, shot, precision=64):
     return out
 
 
+def rtm_splitpml(p, vp, vs, shot, precision=64, recon_err=False):
+    """RTM with the first-order split-PML engine: images, Laplacians and,
+    with recon_err, max |Vpx_recon - Vpx_fwd| / max |Vpx_fwd| per level."""
+    m = _ModelRef(vp, vs)
+    s = _ShotRef(**shot)
+    outs = [np.zeros((p.nx, p.nz), np.float64) for _ in range(4)]
+    err = np.zeros(p.nt, np.float64) if recon_err else None
+    _check(lib().oracle_rtm_splitpml(C.byref(p), C.byref(m.m), C.byref(s.s), precision,
+                                     *[abi.dptr(a) for a in outs], abi.dptr(err)), "split-PML rtm")
+    out = {"img_pp": outs[0], "img_ps": outs[1], "lap_pp": outs[2], "lap_ps": outs[3]}
+    if recon_err:
+        out["recon_err"] = err
+    return out
+
+
 def fwi_gradient(p, vp, vs, shot, observed, precision=64):
     """gradfun accumulated over one shot's backward loop, driven by the
     residual modelled - observed (adsource); observed is [2][nx][nt]."""
@@ -262,8 +278,10 @@ def laplace(img, dh):
 
 
 class mutation:
-    """Context manager planting mistake u (1..12: ax/az swapped in one CPML
-    memory-variable update) in the oracle, for pins that must catch it."""
+    """Context manager planting mistake u in the oracle, for pins that must
+    catch it (1..12: ax/az swapped in one CPML memory-variable update; 13, 14:
+    the split-PML reconstruction's ring velocities one slot late, its source
+    sample dropped)."""
 
     def __init__(self, u):
         self.u = int(u)

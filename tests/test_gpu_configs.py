@@ -1,0 +1,171 @@
+"""The two full-size RTM configurations on the GPU against committed FP64
+oracle images (tests/golden/make_config_fixtures.py writes them):
+
+  rtm1024 - plain single-shot RTM, 1024 x 512 at 5 m, 4000 steps, FP32
+            strips, one shot at x = 512; the plain engine (the reference's
+            kernel structure) and the fused engine;
+  rtm2048 - batched RTM, 2048 x 1024 at 5 m, 6000 steps, scaled-FP16 strips,
+            fused engine, shots s = 0 (x = 16) and s = 32 (x = 1040) of the
+            64-shot layout, and the stack of the two.
+
+Every image (PP, PS and their Laplacians, the outputs the reference writes,
+P-S-seperating.cu:2367-2400) is held to the 1e-4 image bar over each region of
+tests/image_metrics.py.  A case the engine misses is marked xfail(strict=True)
+with its measured error, so an improvement shows as an unexpected pass.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from image_metrics import IMAGE_BAR, KINDS, REGIONS, bar_param, errors_fixture, rel
+from ouc_amd import abi, velocity_models as vm
+from ouc_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# Measured errors of the cases that miss the bar: (run, shot, kind, region) -> relative L2.
+MISSES = {
+    ('rtm1024-plain', 0, 'img_pp', 'whole'): 2.93e-03,
+    ('rtm1024-plain', 0, 'img_pp', 'top'): 7.41e-03,
+    ('rtm1024-plain', 0, 'img_ps', 'whole'): 8.48e-03,
+    ('rtm1024-plain', 0, 'img_ps', 'below8'): 5.10e-04,
+    ('rtm1024-plain', 0, 'img_ps', 'near'): 4.62e-04,
+    ('rtm1024-plain', 0, 'img_ps', 'top'): 7.16e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'whole'): 2.84e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'below8'): 2.78e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'near'): 2.78e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'top'): 2.18e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'whole'): 2.79e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'below8'): 2.02e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'near'): 2.02e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'top'): 2.17e-03,
+    ('rtm1024-fused', 0, 'img_pp', 'whole'): 1.26e-03,
+    ('rtm1024-fused', 0, 'img_pp', 'top'): 3.00e-03,
+    ('rtm1024-fused', 0, 'img_ps', 'whole'): 4.39e-03,
+    ('rtm1024-fused', 0, 'img_ps', 'below8'): 5.48e-04,
+    ('rtm1024-fused', 0, 'img_ps', 'near'): 5.26e-04,
+    ('rtm1024-fused', 0, 'img_ps', 'top'): 4.39e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'whole'): 1.50e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'below8'): 3.31e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'near'): 3.31e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'top'): 1.33e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'whole'): 1.47e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'below8'): 2.42e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'near'): 2.42e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'top'): 1.32e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'whole'): 3.11e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'below8'): 5.67e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'top'): 3.10e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'whole'): 9.97e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'below8'): 6.27e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'near'): 7.69e-04,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'top'): 9.31e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'whole'): 5.00e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'below8'): 7.99e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'near'): 9.08e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'top'): 2.57e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'whole'): 4.97e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'below8'): 7.96e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'near'): 5.86e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'top'): 2.57e-03,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'whole'): 5.86e-02,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'below8'): 3.39e-01,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'top'): 5.87e-02,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'whole'): 2.08e-02,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'below8'): 3.50e-01,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'near'): 8.12e-04,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'top'): 3.51e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'whole'): 8.86e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'below8'): 1.42e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'near'): 9.39e-03,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'top'): 1.18e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'whole'): 9.00e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'below8'): 1.37e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'near'): 5.15e-03,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'top'): 1.18e-02,
+    ('stack', 'pp', 'whole'): 7.51e-02,
+    ('stack', 'pp', 'near'): 9.89e-02,
+    ('stack', 'ps', 'whole'): 7.87e-02,
+    ('stack', 'ps', 'near'): 9.72e-02,
+}
+
+RUNS = {
+    "rtm1024-plain": ("rtm1024", abi.ENGINE_PLAIN, abi.STRIP_FP32, (0,)),
+    "rtm1024-fused": ("rtm1024", abi.ENGINE_FUSED, abi.STRIP_FP32, (0,)),
+    "rtm2048-fused-fp16": ("rtm2048", abi.ENGINE_FUSED, abi.STRIP_FP16, (0, 32)),
+}
+
+_cache = {}
+
+
+def _fixture(config):
+    return np.load(os.path.join(GOLDEN, f"{config}_fp64.npz"))
+
+
+def _run(name):
+    if name in _cache:
+        return _cache[name]
+    config, engine, strips, shots = RUNS[name]
+    if config == "rtm1024":
+        p, vp, vs, shot = vm.cfg_rtm_1024()
+        shot_of = {0: shot}
+    else:
+        p, vp, vs, all_shots = vm.cfg_rtm_2048()
+        shot_of = {s: all_shots[s] for s in shots}
+    p.engine = engine
+    p.strip_precision = strips
+    out = {}
+    with Engine(p, vp, vs) as e:
+        for s in shots:
+            print(f"[case] {name} shot {s} (x = {shot_of[s]['sx']}) {p.nx}x{p.nz} nt={p.nt}", flush=True)
+            out[s] = e.rtm_shot(shot_of[s])
+        if len(shots) > 1:
+            print(f"[case] {name} batch of shots {shots}", flush=True)
+            out["stack"] = e.rtm_batch([shot_of[s] for s in shots])
+    fix = _fixture(config)
+    errs = {s: errors_fixture(out[s], fix, s, p.dx) for s in shots}
+    for s in shots:
+        print(f"[{name} shot {s}] " + " ".join(f"{k}/{r}={v:.2e}" for (k, r), v in errs[s].items()), flush=True)
+    _cache[name] = (out, errs, fix)
+    return _cache[name]
+
+
+def _cases():
+    for name, (_, _, _, shots) in RUNS.items():
+        for s in shots:
+            for k in KINDS:
+                for r in REGIONS:
+                    yield bar_param((name, s, k, r), MISSES, IMAGE_BAR, name, s, k, r, id=f"{name}-shot{s}-{k}-{r}")
+
+
+@pytest.mark.parametrize("name,shot,kind,region", list(_cases()))
+def test_config_image_against_fp64(name, shot, kind, region):
+    _, errs, _ = _run(name)
+    err = errs[shot][(kind, region)]
+    print(f"[{name} shot {shot}] {kind} {region}: {err:.2e}", flush=True)
+    assert err < IMAGE_BAR, err
+
+
+@pytest.mark.parametrize("k,region", [bar_param(("stack", k, r), MISSES, IMAGE_BAR, k, r, id=f"{k}-{r}")
+                                      for k in ("pp", "ps") for r in ("whole", "near")])
+def test_batched_stack_against_fp64(k, region):
+    """The stack of the batched run (the sum of the Laplacian images,
+    P-S-seperating.cu:2403-2443) against the sum of the FP64 Laplacian images,
+    and bit for bit against the sum of the engine's own single-shot images."""
+    out, _, fix = _run("rtm2048-fused-fp16")
+    ddx, ddz = (int(v) for v in fix["decimation"])
+    own = out[0][f"lap_{k}"] + out[32][f"lap_{k}"]
+    assert np.array_equal(out["stack"][f"stack_{k}"], own, equal_nan=True)
+    ref = fix[f"shot0_lap_{k}"].astype(np.float64) + fix[f"shot32_lap_{k}"]
+    a = out["stack"][f"stack_{k}"][::ddx, ::ddz]
+    x = np.arange(ref.shape[0])[:, None] * ddx
+    z = np.arange(ref.shape[1])[None, :] * ddz
+    m = (z >= 2) & (x >= 2) & (x < ref.shape[0] * ddx - 2)
+    if region == "near":
+        m = m & (z >= 8) & ((np.abs(x - 16) < 512) | (np.abs(x - 1040) < 512))
+    err = rel(a, ref, m)
+    print(f"[stack {k} {region}] {err:.2e}", flush=True)
+    assert err < IMAGE_BAR, err

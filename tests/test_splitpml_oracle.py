@@ -46,3 +46,24 @@ def test_homogeneous_explosive_source_has_no_s_part(oracle):
     ap = np.abs(r[2:4]).max()
     a_s = np.abs(r[4:6]).max()
     assert ap > 0 and a_s < 1e-12 * ap, (a_s, ap)
+
+
+def test_splitpml_rtm_reconstruction_reverses_the_forward_run(oracle):
+    """RTM with the first-order engine: the reconstruction (the interior
+    velocity and stress passes reversed, P-S-seperating.cu:1066-1229, with the
+    ring of level t+1 stress and level t velocity restored) returns the
+    forward run's P velocity at every level, to FP64 rounding.  Restoring the
+    ring velocities one slot late (the reference's live-engine defect B3) or
+    dropping the source sample from the reversed stress pass breaks it."""
+    from ouc_amd import velocity_models as vm
+
+    vp, vs, _ = vm.layered_random(60, 40, seed=3)
+    p = abi.make_params(60, 40, 12, 5, 400, 5.0, 5.0, 0.0004, R=1e-3)
+    shot = dict(sx=30, sz=14, rec_z=0, fm=25.0, mute=1)  # below the ring (interior rows 0..N-1)
+    r = oracle.rtm_splitpml(p, vp, vs, shot, 64, recon_err=True)
+    assert r["recon_err"].max() < 1e-12
+    assert np.isfinite(r["img_pp"]).all() and np.abs(r["img_pp"]).max() > 0
+    for u in (13, 14):
+        with oracle.mutation(u):
+            bad = oracle.rtm_splitpml(p, vp, vs, shot, 64, recon_err=True)
+        assert bad["recon_err"].max() > 1e-6, (u, bad["recon_err"].max())
