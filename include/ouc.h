@@ -77,9 +77,13 @@ enum { OUC_LAP_STENCIL = 0, OUC_LAP_FFT5 = 1, OUC_LAP_FFT_CONT = 2 };
  * SPLIT_PML is the reference's other formulation, defined but never launched
  * by its main(): first-order velocity-stress with split-field PML damping
  * (stress P-S-seperating.cu:487-555, velocity 390-485; 12th-order
- * coefficients to five digits at N = 6).  It serves ouc_forward_shot only
- * (CPML profiles of the context, no mute); the RTM and FWI calls return
- * OUC_UNSUPPORTED for it. */
+ * coefficients to five digits at N = 6), with the context's CPML profiles.
+ * ouc_forward_shot runs it (no mute).  ouc_rtm_shot and ouc_rtm_batch run its
+ * backward half too (receiver run with velocity_backward_propagation /
+ * stress_backward_propagation, P-S-seperating.cu:728-887; source
+ * reconstruction with velocity_ / stress_wavefield_reconstruction,
+ * P-S-seperating.cu:1066-1229, from FP32 strips), with EXACT time alignment
+ * only; the FWI calls return OUC_UNSUPPORTED for it. */
 enum { OUC_ENGINE_PLAIN = 0, OUC_ENGINE_FUSED = 1, OUC_ENGINE_SPLIT_PML = 2 };
 
 /* Run parameters.  Replaces the #defines L, N, NT, Xn, Zn, R

@@ -268,6 +268,18 @@ void apply_mute(ouc_ctx *c, const ouc_shot *s, int ncomp) {
     OUC_HIP(hipGetLastError());
 }
 
+// The caller's records [2][nx][nt] (vx, vz) in place of the forward run's,
+// into c->rec's [component][nt][nx] layout.
+void upload_records(ouc_ctx *c, const float *rec_in) {
+    const Grid &g = c->g;
+    std::vector<float> tr((size_t)2 * g.nt * g.nx);
+    for (int q = 0; q < 2; ++q)
+        for (int ix = 0; ix < g.nx; ++ix)
+            for (int t = 0; t < g.nt; ++t)
+                tr[((size_t)q * g.nt + t) * g.nx + ix] = rec_in[((size_t)q * g.nx + ix) * g.nt + t];
+    copy_h2d(c->stream, c->rec.ptr, tr.data(), tr.size() * sizeof(float));
+}
+
 // Backward loop (P-S-seperating.cu:2313-2364): source reconstruction from the
 // strips, receiver propagation with injection, imaging condition.
 void run_backward_plain(ouc_ctx *c, const ouc_shot *s, const std::vector<float> &wav) {
@@ -504,7 +516,28 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");
     return guarded(c, [&] {
         check_shot(c, s);
-        if (c->p.engine == OUC_ENGINE_SPLIT_PML) throw Error(OUC_UNSUPPORTED, "the split-PML engine models forward only");
+        if (c->p.engine == OUC_ENGINE_SPLIT_PML) {
+            check_full_length(c, s);
+            const Grid &g = c->g;
+            std::vector<float> wav = shot_wavelet(c, s);
+            splitpml_rtm(c, s, wav, c->rec.f(), [&] {
+                if (rec_in) upload_records(c, rec_in);
+                if (s->mute) apply_mute(c, s, 2);
+            }, c->npp.f(), c->nps.f(), c->den.f());
+            make_images(c);
+            OUC_HIP(hipEventRecord(c->ev[3], c->stream));
+            const size_t ni = (size_t)g.nx * g.nz;
+            download(c, img_pp, c->img[0], ni);
+            download(c, img_ps, c->img[1], ni);
+            download(c, img_pp_lap, c->img[2], ni);
+            download(c, img_ps_lap, c->img[3], ni);
+            OUC_HIP(hipStreamSynchronize(c->stream));
+            c->last.forward_ms = ms_between(c->ev[0], c->ev[1]);
+            c->last.backward_ms = ms_between(c->ev[1], c->ev[2]);
+            c->last.image_ms = ms_between(c->ev[2], c->ev[3]);
+            c->last.total_ms = ms_between(c->ev[0], c->ev[3]);
+            return;
+        }
         if (c->p.engine == OUC_ENGINE_FUSED) {
             if (rec_in) check_full_length(c, s);
             fused_rtm_one(c, s, rec_in, img_pp, img_ps, img_pp_lap, img_ps_lap);
@@ -520,16 +553,7 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
         c->F.zero(c->stream, g);
         OUC_HIP(hipEventRecord(c->ev[0], c->stream));
         run_forward_plain(c, s, wav, true, 2, probe);
-        if (rec_in) {
-            std::vector<float> tr((size_t)2 * g.nt * g.nx);
-            for (int q = 0; q < 2; ++q)
-                for (int ix = 0; ix < g.nx; ++ix)
-                    for (int t = 0; t < g.nt; ++t)
-                        tr[((size_t)q * g.nt + t) * g.nx + ix] = rec_in[((size_t)q * g.nx + ix) * g.nt + t];
-            OUC_HIP(hipMemcpyAsync(c->rec.ptr, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice,
-                                   c->stream));
-            OUC_HIP(hipStreamSynchronize(c->stream));
-        }
+        if (rec_in) upload_records(c, rec_in);
         if (s->mute) apply_mute(c, s, 2);
         OUC_HIP(hipEventRecord(c->ev[1], c->stream));
         run_backward_plain(c, s, wav);

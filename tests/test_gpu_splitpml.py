@@ -75,3 +75,77 @@ def test_splitpml_refuses_the_mute():
         with pytest.raises(OucError) as err:
             e.forward_shot(dict(sx=32, sz=2, rec_z=2, fm=30.0, mute=1))
         assert err.value.status == abi.OUC_UNSUPPORTED
+
+
+# ---- RTM with the first-order engine (EXACT alignment, FP32 strips) ---------
+SP_RTM_MISSES = {
+    ("img_pp", "top"): 1.80e-04,
+    ("img_ps", "top"): 4.35e-04,
+    ("lap_pp", "whole"): 6.77e-04,
+    ("lap_pp", "below8"): 5.03e-04,
+    ("lap_pp", "near"): 5.03e-04,
+    ("lap_pp", "top"): 7.88e-03,
+    ("lap_ps", "whole"): 2.88e-04,
+    ("lap_ps", "below8"): 2.12e-04,
+    ("lap_ps", "near"): 2.12e-04,
+    ("lap_ps", "top"): 1.50e-03,
+}
+_rtm = {}
+
+
+def _splitpml_rtm(oracle):
+    if not _rtm:
+        from image_metrics import errors_full
+
+        nx, nz, nt = 160, 96, 900
+        vp, vs, _ = vm.layered_random(nx, nz, seed=3)
+        p = abi.make_params(nx, nz, 24, 5, nt, 5.0, 5.0, 0.0004, R=1e-3, semantics=abi.SEM_EXACT,
+                            engine=abi.ENGINE_SPLIT_PML)
+        shot = dict(sx=80, sz=0, rec_z=0, fm=22.0, mute=1)
+        ref = oracle.rtm_splitpml(p, vp, vs, shot, 64)
+        r32 = oracle.rtm_splitpml(p, vp, vs, shot, 32)
+        print(f"[case] split-PML rtm {nx}x{nz} nt={nt} exact", flush=True)
+        with Engine(p, vp, vs) as e:
+            img = e.rtm_shot(shot)
+        _rtm["gpu"] = errors_full(img, ref, shot["sx"])
+        _rtm["fp32"] = errors_full(r32, ref, shot["sx"])
+    return _rtm
+
+
+def _sp_rtm_cases():
+    from image_metrics import IMAGE_BAR, KINDS, REGIONS, bar_param
+
+    for k in KINDS:
+        for r in REGIONS:
+            yield bar_param((k, r), SP_RTM_MISSES, IMAGE_BAR, k, r, id=f"{k}-{r}")
+
+
+@pytest.mark.parametrize("kind,region", list(_sp_rtm_cases()))
+def test_splitpml_rtm_against_oracle(oracle, kind, region):
+    """RTM with the first-order engine (receiver run with the damped kernels,
+    reconstruction by the reversed interior passes from FP32 strips, imaging
+    of source level t+1), 160 x 96, 900 steps, against the FP64 oracle at the
+    image bar; the FP32 oracle's error is printed beside it."""
+    from image_metrics import IMAGE_BAR
+
+    r = _splitpml_rtm(oracle)
+    err = r["gpu"][(kind, region)]
+    print(f"[split-PML rtm] {kind} {region}: {err:.2e} (FP32 oracle {r['fp32'][(kind, region)]:.2e})", flush=True)
+    assert err < IMAGE_BAR, err
+
+
+@pytest.mark.parametrize("what", ["faithful", "fp16", "fwi"])
+def test_splitpml_rtm_refusals(what):
+    p = abi.make_params(64, 48, 12, 5, 60, 5.0, 5.0, 0.0005, R=1e-3, engine=abi.ENGINE_SPLIT_PML,
+                        semantics=abi.SEM_FAITHFUL if what == "faithful" else abi.SEM_EXACT,
+                        strip_precision=abi.STRIP_FP16 if what == "fp16" else abi.STRIP_FP32)
+    vp = np.full((64, 48), 2000.0, np.float32)
+    vs = np.full((64, 48), 1100.0, np.float32)
+    shot = dict(sx=32, sz=0, rec_z=0, fm=30.0, mute=0)
+    with Engine(p, vp, vs) as e:
+        with pytest.raises(OucError) as err:
+            if what == "fwi":
+                e.fwi_gradient([shot], np.zeros((1, 2, 64, 60), np.float32))
+            else:
+                e.rtm_shot(shot)
+    assert err.value.status == abi.OUC_UNSUPPORTED
