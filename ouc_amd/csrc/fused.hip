@@ -84,8 +84,9 @@ struct FusedState {
     DevBuf tiles_s, tiles_v, tiles_rs, tiles_rv;
     int ntiles_s = 0, ntiles_v = 0, ntiles_rs = 0, ntiles_rv = 0;
     bool edge = false;     // CPML band by the streaming k_edge (CPML contexts) instead of the tile kernels
-    EPlan eplan{};
-    int eblocks = 0;
+    // streaming band plans: [0] the source's forward run, [1] the receiver run
+    EPlan eplan[2]{};
+    int eblocks[2] = {0, 0};
     DevBuf img;            // [4][nx*nz] scratch for one shot's images
     // learned boundary
     DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
@@ -343,20 +344,28 @@ void ensure(ouc_ctx *c, int shots) {
                               {N, g.Xn - N, f.cz1, g.Zn - N},
                               {N, f.cx0, f.cz0, f.cz1},
                               {f.cx1, g.Xn - N, f.cz0, f.cz1}};
-        EPlan &pl = st->eplan;
-        pl.xchunk = 64;
-        const int TZ = kEdgeW - 4 * N;
-        int nb = 0;
-        for (int r = 0; r < 4; ++r) {
-            ERect &R = pl.r[r];
-            R.x0 = rx[r][0]; R.x1 = rx[r][1]; R.z0 = rx[r][2]; R.z1 = rx[r][3];
-            const bool any = R.x1 > R.x0 && R.z1 > R.z0;
-            R.nzb = any ? (R.z1 - R.z0 + TZ - 1) / TZ : 1;
-            R.nxb = any ? (R.x1 - R.x0 + pl.xchunk - 1) / pl.xchunk : 0;
-            R.first = nb;
-            nb += R.nzb * R.nxb;
+        // The source's forward steps stream 32-column chunks: at a few shots
+        // per batch the band has under one wave per SIMD and its latency is
+        // exposed, and twice the blocks of half the length run it about 10 %
+        // faster.  The receiver's band runs beside the reconstruction on the
+        // other stream and keeps 64 (fewer halo columns re-read).
+        const int chunk[2] = {32, 64};
+        for (int v = 0; v < 2; ++v) {
+            EPlan &pl = st->eplan[v];
+            pl.xchunk = chunk[v];
+            const int TZ = kEdgeW - 4 * N;
+            int nb = 0;
+            for (int r = 0; r < 4; ++r) {
+                ERect &R = pl.r[r];
+                R.x0 = rx[r][0]; R.x1 = rx[r][1]; R.z0 = rx[r][2]; R.z1 = rx[r][3];
+                const bool any = R.x1 > R.x0 && R.z1 > R.z0;
+                R.nzb = any ? (R.z1 - R.z0 + TZ - 1) / TZ : 1;
+                R.nxb = any ? (R.x1 - R.x0 + pl.xchunk - 1) / pl.xchunk : 0;
+                R.first = nb;
+                nb += R.nzb * R.nxb;
+            }
+            st->eblocks[v] = nb;
         }
-        st->eblocks = nb;
     }
     st->tiles_s.alloc(ts.size() * sizeof(int2));
     st->tiles_v.alloc(tv.size() * sizeof(int2));
@@ -488,12 +497,13 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
     if (f.lpad)
         k_lpad_stress<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a);
     if (st->edge) {
-        EPlan pl = st->eplan;
+        const int v = a.inj ? 1 : 0; // the receiver run injects records
+        EPlan pl = st->eplan[v];
         pl.save_ring = save_ring ? 1 : 0;
         if (a.spx) {
-            OUC_FDISPATCH(f.N, k_edge<NN, true><<<dim3(st->eblocks, 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
+            OUC_FDISPATCH(f.N, k_edge<NN, true><<<dim3(st->eblocks[v], 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
         } else {
-            OUC_FDISPATCH(f.N, k_edge<NN, false><<<dim3(st->eblocks, 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
+            OUC_FDISPATCH(f.N, k_edge<NN, false><<<dim3(st->eblocks[v], 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
         }
     } else {
         OUC_FDISPATCH(f.N, k_band_stress<NN><<<L.band_s, dim3(64, 4), 0, s>>>(f, w, km, st->tiles_s.as<int2>(), a));
