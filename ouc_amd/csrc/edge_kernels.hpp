@@ -64,12 +64,11 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     const float *c = g.c;
     const int wb = 1 - a.rb;
     const unsigned P4 = (unsigned)g.P * 4u, zb4 = (unsigned)z * 4u;
-    auto off = [&](int col) { return (unsigned)col * P4 + zb4; };
-    const float *const Vr0 = w.V[a.rb][0] + so, *const Vr1 = w.V[a.rb][1] + so;
-    float *const Vw0 = w.V[wb][0] + so, *const Vw1 = w.V[wb][1] + so;
-    float *const DV0 = w.DV[0] + so, *const DV1 = w.DV[1] + so;
-    float *const S0 = w.S[0] + so, *const S1 = w.S[1] + so;
-    float *const DS0 = w.DS[0] + so, *const DS1 = w.DS[1] + so;
+    auto off = [&](int col) { return (unsigned)col * P4 + zb4; };        // float arrays
+    auto off2 = [&](int col) { return 2u * ((unsigned)col * P4 + zb4); }; // pair arrays
+    const float2 *const Vr = w.V[a.rb].p + so;
+    float2 *const Vw = w.V[wb].p + so;
+    float2 *const DVp = w.DV.p + so, *const Sp = w.S.p + so, *const DSp = w.DS.p + so;
     float *const *Fr = a.rb == 0 ? w.F : w.F2; // F of the previous step
     float *const *Fw = a.rb == 0 ? w.F2 : w.F; // F of this step (owner writes)
     const float *const Fr0 = Fr[0] + so, *const Fr1 = Fr[1] + so, *const Fr2 = Fr[2] + so, *const Fr3 = Fr[3] + so;
@@ -85,7 +84,7 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
     const bool img_on = IMG && !(a.t0 && a.t < a.t0[blockIdx.z]);
     const bool z_int = z >= g.L && z < g.Zn - g.L;
-    const float *const spx = img_on ? a.spx + so : nullptr, *const spz = img_on ? a.spz + so : nullptr;
+    const float2 *const sp = img_on ? a.sp + so : nullptr;
     float *const npp = img_on ? a.npp + so : nullptr, *const nps = img_on ? a.nps + so : nullptr;
     float *const den = img_on ? a.den + so : nullptr;
 
@@ -93,9 +92,9 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
         vx = 0.f;
         vz = 0.f;
         if (!z_vin || i < g.vlo || i >= g.Xn - g.vlo) return;
-        const unsigned o = off(i);
-        vx = ldb(Vr0, o);
-        vz = ldb(Vr1, o);
+        const float2 v = ldb2(Vr, off2(i));
+        vx = v.x;
+        vz = v.y;
         if (z_inj && i >= g.L && i < g.Xn - g.L) { // load_record, P-S-seperating.cu:1471-1483
             long io = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
             vx = a.inj[io];
@@ -116,15 +115,16 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     // update operands at column cu: state, O, ax, az (+ imaging)
     auto load_uop = [&](int cu, float *o) {
         if (z_out && cu >= xs && cu < xe) {
-            const unsigned k = off(cu);
-            o[0] = ldb(DV0, k); o[1] = ldb(DV1, k); o[2] = ldb(S0, k); o[3] = ldb(S1, k);
-            o[4] = ldb(DS0, k); o[5] = ldb(DS1, k);
+            const unsigned k2 = off2(cu), k = off(cu);
+            float2 v = ldb2(DVp, k2); o[0] = v.x; o[1] = v.y;
+            v = ldb2(Sp, k2); o[2] = v.x; o[3] = v.y;
+            v = ldb2(DSp, k2); o[4] = v.x; o[5] = v.y;
             o[6] = ldb(O0, k); o[7] = ldb(O1, k); o[8] = ldb(O2, k); o[9] = ldb(O3, k);
             o[10] = ldb(O4, k); o[11] = ldb(O5, k); o[12] = ldb(O6, k); o[13] = ldb(O7, k);
             o[14] = ldb(m.ax, k); o[15] = ldb(m.az, k);
             if (IMG) {
                 if (img_on && z_int && cu >= g.L && cu < g.Xn - g.L) {
-                    o[16 % NS] = ldb(spx, k); o[17 % NS] = ldb(spz, k); o[18 % NS] = ldb(npp, k);
+                    v = ldb2(sp, k2); o[16 % NS] = v.x; o[17 % NS] = v.y; o[18 % NS] = ldb(npp, k);
                     o[19 % NS] = ldb(den, k); o[20 % NS] = ldb(nps, k);
                 } else {
                     o[16 % NS] = o[17 % NS] = o[18 % NS] = o[19 % NS] = o[20 % NS] = 0.f;
@@ -284,15 +284,17 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
             const float Kx = (KX - FX) - OC + OE, Kz = (KZ - FZ) - OB + OA;
             const float hx = Wz - OWz, hz = -Wx + OWx;
             const float dt2 = g.dt2;
+            const unsigned k2 = off2(cu);
             float vx = qvx[RS(0)], vz = qvz[RS(0)];
-            if (z_inj) { vx = ldb(Vr0, k); vz = ldb(Vr1, k); } // state, not the injected record
-            float d;
-            d = uo[0] + dt2 * ((Tx - OTx) + hx + 2.f * Kx); stb(DV0, k, d); vx = vx + d;
-            d = uo[1] + dt2 * ((Tz - OTz) + hz + 2.f * Kz); stb(DV1, k, d); vz = vz + d;
-            stb(Vw0, k, vx);
-            stb(Vw1, k, vz);
-            d = uo[4] + dt2 * (hx + Kx); stb(DS0, k, d); const float sx = uo[2] + d; stb(S0, k, sx);
-            d = uo[5] + dt2 * (hz + Kz); stb(DS1, k, d); const float sz = uo[3] + d; stb(S1, k, sz);
+            if (z_inj) { const float2 v = ldb2(Vr, k2); vx = v.x; vz = v.y; } // state, not the injected record
+            const float dx = uo[0] + dt2 * ((Tx - OTx) + hx + 2.f * Kx); vx = vx + dx;
+            const float dz = uo[1] + dt2 * ((Tz - OTz) + hz + 2.f * Kz); vz = vz + dz;
+            stb2(DVp, k2, make_float2(dx, dz));
+            stb2(Vw, k2, make_float2(vx, vz));
+            const float ex = uo[4] + dt2 * (hx + Kx); const float sx = uo[2] + ex;
+            const float ez = uo[5] + dt2 * (hz + Kz); const float sz = uo[3] + ez;
+            stb2(DSp, k2, make_float2(ex, ez));
+            stb2(Sp, k2, make_float2(sx, sz));
             record(g, a, cu, z, vx, vz, sx, sz);
             if (IMG && img_on && z_int && cu >= g.L && cu < g.Xn - g.L) { // corr_v, P-S-seperating.cu:1426-1439
                 const float sa = uo[16 % NS], sb = uo[17 % NS];

@@ -440,6 +440,12 @@ ouc_status ouc_create(const ouc_params *p, const ouc_model *m, int32_t device, o
         copy_h2d(c->stream, c->q2.ptr, q2.data(), pn);
         copy_h2d(c->stream, c->ax.ptr, ax.data(), pn);
         copy_h2d(c->stream, c->az.ptr, az.data(), pn);
+        {   // (vp^2, vs^2) interleaved: the fused core kernel reads both with one 8-byte load
+            std::vector<float> pq(2 * p2.size());
+            for (size_t k = 0; k < p2.size(); ++k) { pq[2 * k] = p2[k]; pq[2 * k + 1] = q2[k]; }
+            c->pq.alloc(2 * pn);
+            copy_h2d(c->stream, c->pq.ptr, pq.data(), 2 * pn);
+        }
         c->F.alloc(g);
         c->R.alloc(g);
         for (auto &lv : c->vp3)

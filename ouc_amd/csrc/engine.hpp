@@ -77,6 +77,7 @@ struct ouc_ctx {
     std::vector<float> vp_in, vs_in; // interior model (the CPML fallback context is built from it)
     ouc_ctx *fallback = nullptr;     // CPML context of a learned-boundary context, made on first need
     ouc::DevBuf p2, q2, ax, az;
+    ouc::DevBuf pq; // (vp^2, vs^2) pairs, interleaved
     ouc::DevWave F, R;
     ouc::DevBuf vp3[3][2];
     ouc::DevBuf npp, nps, den;

@@ -39,29 +39,29 @@ namespace {
 // k_core with the prefetch depth chosen at run time (1..3 for N = 5, 6; the
 // default elsewhere, to bound the number of instantiations).
 template <int N, int MODE>
-void launch_core_mode(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2,
-                      const float *q2, const FStep &a, const FRecon &rc) {
+void launch_core_mode(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float2 *pq,
+                      const FStep &a, const FRecon &rc) {
     if constexpr (N == 5 || N == 6) {
         switch (pf) {
-        case 1: k_core<N, MODE, 1><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
-        case 3: k_core<N, MODE, 3><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc); return;
+        case 1: k_core<N, MODE, 1><<<grid, 256, 0, s>>>(f, w, pq, a, rc); return;
+        case 3: k_core<N, MODE, 3><<<grid, 256, 0, s>>>(f, w, pq, a, rc); return;
         default: break;
         }
     }
-    k_core<N, MODE, kCorePF><<<grid, 256, 0, s>>>(f, w, p2, q2, a, rc);
+    k_core<N, MODE, kCorePF><<<grid, 256, 0, s>>>(f, w, pq, a, rc);
 }
 
 // RECON = 1: the reconstruction; otherwise a wavefield step, with the imaging
-// condition when the step carries imaging operands (a.spx).  The tuned
+// condition when the step carries imaging operands (a.sp).  The tuned
 // prefetch depth applies to the plain step; the imaging mode carries more
 // operands per column and runs with one column in flight (rotating queues, no
 // register spills), the reconstruction with two (the deepest without spills).
 template <int N, int RECON>
-void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float *p2, const float *q2,
-                 const FStep &a, const FRecon &rc) {
-    if constexpr (RECON) launch_core_mode<N, CORE_RECON>(2, grid, s, f, w, p2, q2, a, rc);
-    else if (a.spx) launch_core_mode<N, CORE_IMAGE>(1, grid, s, f, w, p2, q2, a, rc);
-    else launch_core_mode<N, CORE_STEP>(pf, grid, s, f, w, p2, q2, a, rc);
+void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float2 *pq, const FStep &a,
+                 const FRecon &rc) {
+    if constexpr (RECON) launch_core_mode<N, CORE_RECON>(2, grid, s, f, w, pq, a, rc);
+    else if (a.sp) launch_core_mode<N, CORE_IMAGE>(1, grid, s, f, w, pq, a, rc);
+    else launch_core_mode<N, CORE_STEP>(pf, grid, s, f, w, pq, a, rc);
 }
 
 constexpr int kWaveArrays = 32; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O, 4 F2
@@ -72,7 +72,7 @@ struct FusedState {
     FGrid g{};
     DevBuf mem_s, mem_r;   // wavefields
     FWave S{}, R{};
-    DevBuf vp[3][2];       // source P velocity levels [cap][plane]
+    DevBuf vp[3];          // source P velocity levels, (x, z) pairs [cap][plane]
     DevBuf acc[3];         // npp, nps, den
     DevBuf strips32, strips16, e8;   // FP32 strips, or FP16 residuals and chunk exponents
     DevBuf keyb, ef, decb;           // FP16: keyframes, encoder state, decoded slots ([cap][2][RS_NF][ring])
@@ -169,16 +169,18 @@ constexpr int kMonStride = 8; // steps between boundary-monitor launches
 
 FusedState *fs(ouc_ctx *c) { return (FusedState *)c->fused; }
 
+// The first five pair arrays (V[0], V[1], DV, S, DS: ten planes) hold (x, z)
+// interleaved; the 22 single arrays follow.
 void wave_layout(DevBuf &mem, FWave &w, long plane, int cap) {
     mem.alloc((size_t)kWaveArrays * plane * cap * sizeof(float));
     float *b = mem.f();
     const size_t span = (size_t)plane * cap;
-    float **slots[kWaveArrays] = {&w.V[0][0], &w.V[0][1], &w.V[1][0], &w.V[1][1], &w.DV[0], &w.DV[1], &w.S[0],
-                                  &w.S[1],    &w.DS[0],   &w.DS[1],   &w.th,      &w.om,     &w.gxx,   &w.gxz,
-                                  &w.gzx,     &w.gzz,     &w.F[0],    &w.F[1],    &w.F[2],   &w.F[3],  &w.O[0],
-                                  &w.O[1],    &w.O[2],    &w.O[3],    &w.O[4],    &w.O[5],   &w.O[6],  &w.O[7],
-                                  &w.F2[0],   &w.F2[1],   &w.F2[2],   &w.F2[3]};
-    for (int a = 0; a < kWaveArrays; ++a) *slots[a] = b + (size_t)a * span;
+    Pair *pairs[5] = {&w.V[0], &w.V[1], &w.DV, &w.S, &w.DS};
+    for (int a = 0; a < 5; ++a) pairs[a]->p = reinterpret_cast<float2 *>(b + (size_t)2 * a * span);
+    float **slots[kWaveArrays - 10] = {&w.th,    &w.om,    &w.gxx,   &w.gxz,   &w.gzx,   &w.gzz,   &w.F[0],  &w.F[1],
+                                       &w.F[2],  &w.F[3],  &w.O[0],  &w.O[1],  &w.O[2],  &w.O[3],  &w.O[4],  &w.O[5],
+                                       &w.O[6],  &w.O[7],  &w.F2[0], &w.F2[1], &w.F2[2], &w.F2[3]};
+    for (int a = 0; a < kWaveArrays - 10; ++a) *slots[a] = b + (size_t)(10 + a) * span;
 }
 
 FGrid make_fgrid(const ouc_ctx *c) {
@@ -298,8 +300,7 @@ void ensure(ouc_ctx *c, int shots) {
     wave_layout(st->mem_s, st->S, st->plane, shots);
     wave_layout(st->mem_r, st->R, st->plane, shots);
     const size_t span = (size_t)st->plane * shots * sizeof(float);
-    for (auto &lv : st->vp)
-        for (auto &b : lv) b.alloc(span);
+    for (auto &lv : st->vp) lv.alloc(2 * span);
     for (auto &b : st->acc) b.alloc(span);
     st->nslots = g.nt;
     st->nchunks = (c->ring.ncells + 63) / 64;
@@ -500,7 +501,7 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
         const int v = a.inj ? 1 : 0; // the receiver run injects records
         EPlan pl = st->eplan[v];
         pl.save_ring = save_ring ? 1 : 0;
-        if (a.spx) {
+        if (a.sp) {
             OUC_FDISPATCH(f.N, k_edge<NN, true><<<dim3(st->eblocks[v], 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
         } else {
             OUC_FDISPATCH(f.N, k_edge<NN, false><<<dim3(st->eblocks[v], 1, B), kEdgeW, 0, s>>>(f, w, km, a, pl));
@@ -511,7 +512,7 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
     if (L.has_core) {
         bool p = pr.want(nlaunch++);
         if (p) OUC_HIP(hipEventRecord(pr.ev[2 * pr.used], s));
-        OUC_FDISPATCH(f.N, launch_core<NN, 0>(st->pf, L.core, s, f, w, c->p2.f(), c->q2.f(), a, FRecon{}));
+        OUC_FDISPATCH(f.N, launch_core<NN, 0>(st->pf, L.core, s, f, w, c->pq.as<float2>(), a, FRecon{}));
         if (p) { OUC_HIP(hipEventRecord(pr.ev[2 * pr.used + 1], s)); ++pr.used; }
     }
     if (!st->edge) {
@@ -531,13 +532,13 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
 }
 
 // Crop of V (both components) of shot 0 into out[2][w][h] (x-major).
-__global__ void k_crop_v(FGrid g, const float *__restrict__ vx, const float *__restrict__ vz, int x0, int z0, int w,
-                         int h, float *__restrict__ out) {
+__global__ void k_crop_v(FGrid g, const float2 *__restrict__ v, int x0, int z0, int w, int h, float *__restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (j >= h || i >= w) return;
     const long k = (long)(x0 + i) * g.P + (z0 + j);
-    out[(long)i * h + j] = vx[k];
-    out[(long)w * h + (long)i * h + j] = vz[k];
+    const float2 vk = v[k];
+    out[(long)i * h + j] = vk.x;
+    out[(long)w * h + (long)i * h + j] = vk.y;
 }
 
 // Wavefield snapshots taken after the listed forward steps (increasing).
@@ -627,7 +628,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         wave_step_probed(c, st, L, st->S, a, probe, nlaunch, mon_f, rtm);
         if (snap && ksnap < snap->n && snap->steps[ksnap] == t) {
             k_crop_v<<<dim3((snap->h + 255) / 256, snap->w), 256, 0, s>>>(
-                f, st->S.V[1 - rb][0], st->S.V[1 - rb][1], snap->x0, snap->z0, snap->w, snap->h,
+                f, st->S.V[1 - rb].p, snap->x0, snap->z0, snap->w, snap->h,
                 snap->dev + (size_t)ksnap * 2 * snap->w * snap->h);
             ++ksnap;
         }
@@ -674,8 +675,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         k_fmute<<<gm, 256, 0, s>>>(g.nx, nt, st->counts.as<int32_t>(), st->rec.f());
     }
     // hand-off: V^nt in buffer rb; V^{nt-1} into 1-rb; source P levels nt, nt-1
-    k_funstep<<<L.full, dim3(64, 4), 0, s>>>(f, st->S, rb, st->vp[nt % 3][0].f(), st->vp[nt % 3][1].f(),
-                                             st->vp[(nt - 1) % 3][0].f(), st->vp[(nt - 1) % 3][1].f());
+    k_funstep<<<L.full, dim3(64, 4), 0, s>>>(f, st->S, rb, st->vp[nt % 3].as<float2>(),
+                                             st->vp[(nt - 1) % 3].as<float2>());
     int rbs = 1 - rb;
     OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
     for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, span, s));
@@ -716,8 +717,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             rc.slot_v = handoff ? -1 : (exact ? t : t + 1);
             rc.slot_th = t + 1;
             rc.exact = (exact || handoff) ? 1 : 0;
-            rc.vpx = fwi ? nullptr : st->vp[t % 3][0].f();
-            rc.vpz = fwi ? nullptr : st->vp[t % 3][1].f();
+            rc.vp = fwi ? nullptr : st->vp[t % 3].as<float2>();
             FStep ra{};
             ra.t = t + 1;
             ra.rb = rbs;
@@ -726,13 +726,13 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
             OUC_FDISPATCH(f.N, k_recon_stress_f<NN><<<L.recon_s, dim3(64, 4), 0, sr>>>(
                                    f, st->S, c->p2.f(), c->q2.f(), rc, ra, st->tiles_rs.as<int2>()));
             if (L.has_recon_core)
-                OUC_FDISPATCH(f.N, launch_core<NN, 1>(st->pf, L.recon_core, sr, L.rg, st->S, c->p2.f(), c->q2.f(), ra, rc));
+                OUC_FDISPATCH(f.N, launch_core<NN, 1>(st->pf, L.recon_core, sr, L.rg, st->S, c->pq.as<float2>(), ra, rc));
             OUC_FDISPATCH(f.N, k_recon_velocity_f<NN><<<L.recon_v, dim3(64, 4), 0, sr>>>(f, st->S, c->q2.f(), rc, ra,
                                                                                          st->tiles_rv.as<int2>()));
             rbs = 1 - rbs;
             if (rsnap && krs < rsnap->n && rsnap->steps[krs] == t) { // reconstructed V^t
                 k_crop_v<<<dim3((rsnap->h + 255) / 256, rsnap->w), 256, 0, sr>>>(
-                    f, st->S.V[rbs][0], st->S.V[rbs][1], rsnap->x0, rsnap->z0, rsnap->w, rsnap->h,
+                    f, st->S.V[rbs].p, rsnap->x0, rsnap->z0, rsnap->w, rsnap->h,
                     rsnap->dev + (size_t)krs * 2 * rsnap->w * rsnap->h);
                 ++krs;
             }
@@ -741,8 +741,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         r.t = t;
         r.rb = rbr;
         // the FWI gradient needs no image: its receiver step runs without the imaging condition
-        r.spx = fwi ? nullptr : st->vp[lev % 3][0].f();
-        r.spz = fwi ? nullptr : st->vp[lev % 3][1].f();
+        r.sp = fwi ? nullptr : st->vp[lev % 3].as<float2>();
         CoreProbe none;
         int nl2 = 0;
         wave_step_probed(c, st, L, st->R, r, none, nl2, mon_r);
@@ -1279,11 +1278,11 @@ void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_
             dim3 grid((f.cx1 - f.cx0 + f.csx - 1) / f.csx, core_zblocks(f.cz0, f.cz1), nshots);
             FStep st0{};
             if (f.cx1 <= f.cx0) { if (ms) ms[k] = 0; continue; }
-            OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->p2.f(), c->q2.f(), st0, FRecon{})); // warm
+            OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->pq.as<float2>(), st0, FRecon{})); // warm
             OUC_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < reps; ++r) {
                 st0.rb = r & 1;
-                OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->p2.f(), c->q2.f(), st0, FRecon{}));
+                OUC_FDISPATCH(f.N, launch_core<NN, 0>(pfs[b], grid, s, f, st->S, c->pq.as<float2>(), st0, FRecon{}));
             }
             OUC_HIP(hipEventRecord(e1, s));
             OUC_HIP(hipEventSynchronize(e1));

@@ -63,10 +63,22 @@ __device__ __forceinline__ bool in_lpad(const FGrid &g, int i, int j) {
     return g.lpad > 0 && edge_dist(g, i, j) < g.lpad;
 }
 
+// (x, z) component pairs stored interleaved: element k of a plane is
+// (x[k], z[k]) at floats 2k and 2k+1, so a lane moves both components of its
+// cell with one 8-byte access.  Pair[f][k] is component f of element k.
+struct Comp {
+    float *b;
+    __device__ __forceinline__ float &operator[](long k) const { return b[2 * k]; }
+};
+struct Pair {
+    float2 *p = nullptr;
+    __host__ __device__ __forceinline__ Comp operator[](int f) const { return Comp{reinterpret_cast<float *>(p) + f}; }
+};
+
 // One wavefield for all shots of a batch; arrays are [shot][plane].
 struct FWave {
-    float *V[2][2];          // [buffer][x, z]
-    float *DV[2], *S[2], *DS[2];
+    Pair V[2];               // [buffer] (vx, vz)
+    Pair DV, S, DS;          // increment of V, S part, its increment
     float *th, *om, *gxx, *gxz, *gzx, *gzz; // band stress (stored)
     float *F[4], *O[8];
     float *F2[4];            // second F buffer (the streaming CPML band kernel double-buffers F)
@@ -82,7 +94,7 @@ struct FStep {
     float *rec;              // FWD: [shot][ncomp][nt][nx] records, or NULL
     int ncomp;
     const float *inj;        // RCV: [shot][2][nt][nx] records to inject, or NULL
-    const float *spx, *spz;  // RCV: source P velocity level [shot][plane], or NULL
+    const float2 *sp;        // RCV: source P velocity level (x, z) [shot][plane], or NULL
     float *npp, *nps, *den;  // RCV: imaging accumulators [shot][plane]
     const int *t0;           // RCV: [shot] first step of each shot (later starts image only their own steps)
 };
@@ -100,8 +112,9 @@ __device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const F
     vz = 0.f;
     if (i < g.vlo || i >= g.Xn - g.vlo || j < g.vlo || j >= g.Zn - g.vlo) return;
     long k = so + (long)i * g.P + j;
-    vx = w.V[a.rb][0][k];
-    vz = w.V[a.rb][1][k];
+    const float2 v = w.V[a.rb].p[k];
+    vx = v.x;
+    vz = v.y;
     if (a.inj && j == a.zr && i >= g.L && i < g.Xn - g.L) {
         long o = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
         vx = a.inj[o];
@@ -134,9 +147,10 @@ __device__ __forceinline__ void record(const FGrid &g, const FStep &a, int i, in
 // corr_v (P-S-seperating.cu:1426-1439) for an interior cell.
 __device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k, int i, int j, float vx, float vz,
                                           float sx, float sz) {
-    if (!a.spx || i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L) return;
+    if (!a.sp || i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L) return;
     if (a.t0 && a.t < a.t0[blockIdx.z]) return;
-    float sa = a.spx[k], sb = a.spz[k];
+    const float2 spk = a.sp[k];
+    float sa = spk.x, sb = spk.y;
     float rpx = vx - sx, rpz = vz - sz;
     a.npp[k] += sa * rpx + sb * rpz;
     a.den[k] += sa * sa + sb * sb;
@@ -158,7 +172,7 @@ struct FRecon {
     int slot_v;          // slot of the ring totals (< 0: the state's own, no override)
     int slot_th;         // slot of the ring stress-like fields
     int exact;           // add the source term (EXACT semantics, or the hand-off step)
-    float *vpx, *vpz;    // [shot][plane] output level, or NULL (no imaging condition)
+    float2 *vp;          // [shot][plane] output P level (x, z), or NULL (no imaging condition)
 };
 
 __device__ __forceinline__ KStrips shot_strips(const FRecon &rc) { return strips_of_shot(rc.strips, rc.stride, blockIdx.z); }
@@ -174,8 +188,9 @@ __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const 
         return;
     }
     long k = so + (long)i * g.P + j;
-    vx = w.V[rb][0][k];
-    vz = w.V[rb][1][k];
+    const float2 v = w.V[rb].p[k];
+    vx = v.x;
+    vz = v.y;
 }
 
 // Output rows of a core block: 224 = 7 lines of 32 floats (128 B), with block
@@ -201,6 +216,12 @@ __device__ __forceinline__ float ldb(const float *base, unsigned boff) {
 __device__ __forceinline__ void stb(float *base, unsigned boff, float v) {
     *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff) = v;
 }
+__device__ __forceinline__ float2 ldb2(const float2 *base, unsigned boff) {
+    return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(base) + boff);
+}
+__device__ __forceinline__ void stb2(float2 *base, unsigned boff, float2 v) {
+    *reinterpret_cast<float2 *>(reinterpret_cast<char *>(base) + boff) = v;
+}
 
 // MODE 0 steps a wavefield (the forward source run, or the receiver run
 // without imaging); MODE 2 is the receiver run with the imaging condition
@@ -212,8 +233,8 @@ __device__ __forceinline__ void stb(float *base, unsigned boff, float v) {
 // off the ring.
 enum { CORE_STEP = 0, CORE_RECON = 1, CORE_IMAGE = 2 };
 template <int N, int MODE, int PF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float *__restrict__ p2,
-                                              const float *__restrict__ q2, FStep a, FRecon rc) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float2 *__restrict__ pq,
+                                              FStep a, FRecon rc) {
     constexpr bool RECON = MODE == CORE_RECON, IMG = MODE == CORE_IMAGE;
     constexpr int NS = IMG ? 11 : 6; // prefetched per-cell operands: state (+ imaging)
     constexpr int TZ = kCoreTZ;
@@ -234,12 +255,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const bool z_inj = a.inj && z == a.zr;
     // per-block bases (shot plane folded in) and lane byte offset
     const unsigned P4 = (unsigned)g.P * 4u, zb = (unsigned)z * 4u;
-    auto off = [&](int col) { return (unsigned)col * P4 + zb; };
-    const float *const Vr0 = w.V[a.rb][0] + so, *const Vr1 = w.V[a.rb][1] + so;
-    float *const Vw0 = w.V[wb][0] + so, *const Vw1 = w.V[wb][1] + so;
-    float *const DV0 = w.DV[0] + so, *const DV1 = w.DV[1] + so;
-    float *const S0 = w.S[0] + so, *const S1 = w.S[1] + so;
-    float *const DS0 = w.DS[0] + so, *const DS1 = w.DS[1] + so;
+    auto off = [&](int col) { return (unsigned)col * P4 + zb; };        // float arrays
+    auto off2 = [&](int col) { return 2u * ((unsigned)col * P4 + zb); }; // pair arrays
+    const float2 *const Vr = w.V[a.rb].p + so;
+    float2 *const Vw = w.V[wb].p + so;
+    float2 *const DVp = w.DV.p + so, *const Sp = w.S.p + so, *const DSp = w.DS.p + so;
     // source cell and wavelet sample of this shot and step, read once
     int src_i = -1, src_j = -1;
     float src_v = 0.f;
@@ -250,7 +270,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     // imaging condition on this step (shots that start late image only their own steps)
     const bool img_on = IMG && !(a.t0 && a.t < a.t0[blockIdx.z]);
-    const float *const spx = img_on ? a.spx + so : nullptr, *const spz = img_on ? a.spz + so : nullptr;
+    const float2 *const sp = img_on ? a.sp + so : nullptr;
     float *const npp = img_on ? a.npp + so : nullptr, *const nps = img_on ? a.nps + so : nullptr;
     float *const den = img_on ? a.den + so : nullptr;
     KStrips rs{};
@@ -263,9 +283,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         vx = 0.f;
         vz = 0.f;
         if (!z_vin || i < g.vlo || i >= g.Xn - g.vlo) return;
-        const unsigned o = off(i);
-        vx = ldb(Vr0, o);
-        vz = ldb(Vr1, o);
+        const float2 v = ldb2(Vr, off2(i));
+        vx = v.x;
+        vz = v.y;
         if (z_inj && i >= g.L && i < g.Xn - g.L) { // load_record, P-S-seperating.cu:1471-1483
             long io = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
             vx = a.inj[io];
@@ -284,18 +304,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     auto load_pq = [&](int cs, float *o) {
         o[0] = o[1] = 0.f;
         if (z_st && cs >= xs - N && cs < xe + N) {
-            const unsigned k = off(cs);
-            o[0] = ldb(p2, k);
-            o[1] = ldb(q2, k);
+            const float2 v = ldb2(pq, off2(cs));
+            o[0] = v.x;
+            o[1] = v.y;
         }
     };
     auto load_state = [&](int cu, float *o) {
         if (z_out && cu >= xs && cu < xe) {
-            const unsigned k = off(cu);
-            o[0] = ldb(DV0, k); o[1] = ldb(DV1, k); o[2] = ldb(S0, k); o[3] = ldb(S1, k);
-            o[4] = ldb(DS0, k); o[5] = ldb(DS1, k);
+            const unsigned k = off2(cu);
+            float2 v = ldb2(DVp, k); o[0] = v.x; o[1] = v.y;
+            v = ldb2(Sp, k); o[2] = v.x; o[3] = v.y;
+            v = ldb2(DSp, k); o[4] = v.x; o[5] = v.y;
             if (IMG && img_on) {
-                o[6] = ldb(spx, k); o[7] = ldb(spz, k); o[8] = ldb(npp, k); o[9] = ldb(den, k); o[10] = ldb(nps, k);
+                v = ldb2(sp, k); o[6] = v.x; o[7] = v.y;
+                const unsigned k4 = off(cu);
+                o[8] = ldb(npp, k4); o[9] = ldb(den, k4); o[10] = ldb(nps, k4);
             }
         } else {
 #pragma unroll
@@ -407,31 +430,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             tx *= g.idx; wx *= g.idx; kxx *= g.idx; kzx *= g.idx;
             const float Kx = zq[0][2] - kxx, Kz = kzx - zq[0][3];
             const float hx = zq[0][1], hz = -wx;
-            const unsigned k = off(cu);
+            const unsigned k = off2(cu);
             float vx = qvx[RS(0)], vz = qvz[RS(0)];
-            if (z_inj) { vx = ldb(Vr0, k); vz = ldb(Vr1, k); } // state, not the injected record
+            if (z_inj) { const float2 v = ldb2(Vr, k); vx = v.x; vz = v.y; } // state, not the injected record
             const float dt2 = g.dt2;
             const float sg = RECON ? -1.f : 1.f; // the reverse step runs the increments backwards
-            float d;
-            d = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); stb(DV0, k, d); vx = vx + sg * d;
-            d = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); stb(DV1, k, d); vz = vz + sg * d;
-            stb(Vw0, k, vx);
-            stb(Vw1, k, vz);
-            d = st6[4] + sg * dt2 * (hx + Kx); stb(DS0, k, d); const float sx = st6[2] + sg * d; stb(S0, k, sx);
-            d = st6[5] + sg * dt2 * (hz + Kz); stb(DS1, k, d); const float sz = st6[3] + sg * d; stb(S1, k, sz);
+            const float dx = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); vx = vx + sg * dx;
+            const float dz = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); vz = vz + sg * dz;
+            stb2(DVp, k, make_float2(dx, dz));
+            stb2(Vw, k, make_float2(vx, vz));
+            const float ex = st6[4] + sg * dt2 * (hx + Kx); const float sx = st6[2] + sg * ex;
+            const float ez = st6[5] + sg * dt2 * (hz + Kz); const float sz = st6[3] + sg * ez;
+            stb2(DSp, k, make_float2(ex, ez));
+            stb2(Sp, k, make_float2(sx, sz));
             if (RECON) {
-                if (rc.vpx) { // the source P level for the imaging condition (none for the FWI gradient)
-                    stb(rc.vpx + so, k, vx - sx);
-                    stb(rc.vpz + so, k, vz - sz);
-                }
+                if (rc.vp) // the source P level for the imaging condition (none for the FWI gradient)
+                    stb2(rc.vp + so, k, make_float2(vx - sx, vz - sz));
             } else {
                 record(g, a, cu, z, vx, vz, sx, sz);
                 if (IMG && img_on) { // corr_v (P-S-seperating.cu:1426-1439); core cells are interior
                     const float sa = st6[6 % NS], sb = st6[7 % NS];
                     const float rpx = vx - sx, rpz = vz - sz;
-                    stb(npp, k, st6[8 % NS] + (sa * rpx + sb * rpz));
-                    stb(den, k, st6[9 % NS] + (sa * sa + sb * sb));
-                    stb(nps, k, st6[10 % NS] + (sa * sx + sb * sz));
+                    const unsigned k4 = off(cu);
+                    stb(npp, k4, st6[8 % NS] + (sa * rpx + sb * rpz));
+                    stb(den, k4, st6[9 % NS] + (sa * sa + sb * sb));
+                    stb(nps, k4, st6[10 % NS] + (sa * sx + sb * sz));
                 }
             }
         }
@@ -596,15 +619,18 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
     const float hx = Wz - OWz, hz = -Wx + OWx;
     const float dt2 = g.dt2;
     const int wb = 1 - a.rb;
-    float *const DV0 = w.DV[0] + so, *const DV1 = w.DV[1] + so, *const S0 = w.S[0] + so, *const S1 = w.S[1] + so;
-    float *const DS0 = w.DS[0] + so, *const DS1 = w.DS[1] + so;
-    float d;
-    d = ldb(DV0, o) + dt2 * ((Tx - OTx) + hx + 2.f * Kx); stb(DV0, o, d); const float vx = ldb(w.V[a.rb][0] + so, o) + d;
-    d = ldb(DV1, o) + dt2 * ((Tz - OTz) + hz + 2.f * Kz); stb(DV1, o, d); const float vz = ldb(w.V[a.rb][1] + so, o) + d;
-    stb(w.V[wb][0] + so, o, vx);
-    stb(w.V[wb][1] + so, o, vz);
-    d = ldb(DS0, o) + dt2 * (hx + Kx); stb(DS0, o, d); const float sx = ldb(S0, o) + d; stb(S0, o, sx);
-    d = ldb(DS1, o) + dt2 * (hz + Kz); stb(DS1, o, d); const float sz = ldb(S1, o) + d; stb(S1, o, sz);
+    const unsigned o2 = 2u * o; // byte offset of the cell in a pair plane
+    float2 *const DVp = w.DV.p + so, *const Sp = w.S.p + so, *const DSp = w.DS.p + so;
+    const float2 dv = ldb2(DVp, o2), v = ldb2(w.V[a.rb].p + so, o2);
+    const float dx = dv.x + dt2 * ((Tx - OTx) + hx + 2.f * Kx); const float vx = v.x + dx;
+    const float dz = dv.y + dt2 * ((Tz - OTz) + hz + 2.f * Kz); const float vz = v.y + dz;
+    stb2(DVp, o2, make_float2(dx, dz));
+    stb2(w.V[wb].p + so, o2, make_float2(vx, vz));
+    const float2 ds = ldb2(DSp, o2), sv = ldb2(Sp, o2);
+    const float ex = ds.x + dt2 * (hx + Kx); const float sx = sv.x + ex;
+    const float ez = ds.y + dt2 * (hz + Kz); const float sz = sv.y + ez;
+    stb2(DSp, o2, make_float2(ex, ez));
+    stb2(Sp, o2, make_float2(sx, sz));
     record(g, a, i, j, vx, vz, sx, sz);
     correlate(g, a, k, i, j, vx, vz, sx, sz);
 }
@@ -706,17 +732,17 @@ __global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, cons
     Wx *= g.idx; Wz *= g.idz; Tx *= g.idx; Tz *= g.idz;
     const float dt2 = g.dt2;
     const int wb = 1 - a.rb;
-    float d;
-    d = w.DV[0][k] - dt2 * (Tx + Wz + 2.f * KX); w.DV[0][k] = d; const float vx = w.V[a.rb][0][k] - d;
-    d = w.DV[1][k] - dt2 * (Tz - Wx + 2.f * KZ); w.DV[1][k] = d; const float vz = w.V[a.rb][1][k] - d;
-    w.V[wb][0][k] = vx;
-    w.V[wb][1][k] = vz;
-    d = w.DS[0][k] - dt2 * (Wz + KX); w.DS[0][k] = d; const float sx = w.S[0][k] - d; w.S[0][k] = sx;
-    d = w.DS[1][k] - dt2 * (-Wx + KZ); w.DS[1][k] = d; const float sz = w.S[1][k] - d; w.S[1][k] = sz;
-    if (rc.vpx) {
-        rc.vpx[k] = vx - sx;
-        rc.vpz[k] = vz - sz;
-    }
+    const float2 dv = w.DV.p[k], v = w.V[a.rb].p[k];
+    const float dx = dv.x - dt2 * (Tx + Wz + 2.f * KX); const float vx = v.x - dx;
+    const float dz = dv.y - dt2 * (Tz - Wx + 2.f * KZ); const float vz = v.y - dz;
+    w.DV.p[k] = make_float2(dx, dz);
+    w.V[wb].p[k] = make_float2(vx, vz);
+    const float2 ds = w.DS.p[k], sv = w.S.p[k];
+    const float ex = ds.x - dt2 * (Wz + KX); const float sx = sv.x - ex;
+    const float ez = ds.y - dt2 * (-Wx + KZ); const float sz = sv.y - ez;
+    w.DS.p[k] = make_float2(ex, ez);
+    w.S.p[k] = make_float2(sx, sz);
+    if (rc.vp) rc.vp[k] = make_float2(vx - sx, vz - sz);
 }
 
 // Strip store from the stored band stress and the written V (forward):
@@ -731,8 +757,9 @@ __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, 
     const long so = (long)blockIdx.z * g.plane;
     const long k = so + (long)i * g.P + j;
     if (with_v) {
-        strip_put(s, slot, RS_VX, q, valid, valid ? w.V[vbuf][0][k] : 0.f);
-        strip_put(s, slot, RS_VZ, q, valid, valid ? w.V[vbuf][1][k] : 0.f);
+        const float2 v = valid ? w.V[vbuf].p[k] : make_float2(0.f, 0.f);
+        strip_put(s, slot, RS_VX, q, valid, v.x);
+        strip_put(s, slot, RS_VZ, q, valid, v.y);
     }
     if (with_v != 1) {
         strip_put(s, slot, RS_TH, q, valid, valid ? w.th[k] : 0.f);
@@ -746,22 +773,18 @@ __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, 
 
 // Hand-off: (V^nt, D) -> (V^{nt-1}, D) into the other V buffer; P velocity
 // of levels nt and nt-1.
-__global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float *__restrict__ vp_nt_x,
-                                                 float *__restrict__ vp_nt_z, float *__restrict__ vp_m1_x,
-                                                 float *__restrict__ vp_m1_z) {
+__global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float2 *__restrict__ vp_nt,
+                                                 float2 *__restrict__ vp_m1) {
     const int j = blockIdx.x * 64 + threadIdx.x, i = blockIdx.y * 4 + threadIdx.y;
     if (i >= g.Xn || j >= g.Zn) return;
     const long k = (long)blockIdx.z * g.plane + (long)i * g.P + j;
-    const float vx = w.V[rb][0][k], vz = w.V[rb][1][k], sx = w.S[0][k], sz = w.S[1][k];
-    vp_nt_x[k] = vx - sx;
-    vp_nt_z[k] = vz - sz;
-    const float vx1 = vx - w.DV[0][k], vz1 = vz - w.DV[1][k], sx1 = sx - w.DS[0][k], sz1 = sz - w.DS[1][k];
-    w.V[1 - rb][0][k] = vx1;
-    w.V[1 - rb][1][k] = vz1;
-    w.S[0][k] = sx1;
-    w.S[1][k] = sz1;
-    vp_m1_x[k] = vx1 - sx1;
-    vp_m1_z[k] = vz1 - sz1;
+    const float2 v = w.V[rb].p[k], sv = w.S.p[k], dv = w.DV.p[k], ds = w.DS.p[k];
+    const float vx = v.x, vz = v.y, sx = sv.x, sz = sv.y;
+    vp_nt[k] = make_float2(vx - sx, vz - sz);
+    const float vx1 = vx - dv.x, vz1 = vz - dv.y, sx1 = sx - ds.x, sz1 = sz - ds.y;
+    w.V[1 - rb].p[k] = make_float2(vx1, vz1);
+    w.S.p[k] = make_float2(sx1, sz1);
+    vp_m1[k] = make_float2(vx1 - sx1, vz1 - sz1);
 }
 
 } // namespace ouc
