@@ -183,6 +183,22 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in,
 ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
                          float *stack_pp_lap, float *stack_ps_lap, float *per_shot_ms);
 
+/* Batched RTM over shots on several grids at once (fused engine): context i
+ * (its own grid and model) runs shots[i][0 .. nshots[i]).  Every kernel of a
+ * time step is one launch over the shots of all contexts, whatever their grid
+ * sizes, with the time loops captured in HIP graphs; contexts with fewer steps
+ * join the loops late and end with the others.  Each context's images equal
+ * what ouc_rtm_batch gives for its own shots.  stack_pp_lap[i], stack_ps_lap[i]
+ * receive context i's stacks (either array, or an entry, may be NULL).  The
+ * contexts must share the device, half-order (5 or 6), boundary kind, strip
+ * precision and semantics; at most 8 contexts.  With LEARNED_FALLBACK,
+ * diverged shots run again on CPML and the call returns
+ * OUC_NUMERIC_FALLBACK with valid stacks.  No counterpart in the reference,
+ * whose shots all share one grid (the shot loop, P-S-seperating.cu:2108-2401,
+ * and its stack, 2403-2443). */
+ouc_status ouc_rtm_grouped(ouc_ctx *const *ctxs, int32_t nctx, const ouc_shot *const *shots,
+                           const int32_t *nshots, float *const *stack_pp_lap, float *const *stack_ps_lap);
+
 /* Profiling mode: when on, the forward loop brackets a sample of the dominant
  * step kernel's launches with HIP events and reports their summed duration in
  * ouc_timings.step_kernel_ms.  No counterpart in the reference, which never

@@ -42,15 +42,15 @@ __device__ __forceinline__ bool in_ring_frame(const FGrid &g, int i, int j) {
 }
 
 template <int N, bool IMG>
-__global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) void k_edge(FGrid g, FWave w, KModel m,
-                                                                                        FStep a, EPlan pl) {
+__device__ __forceinline__ void edge_body(const FGrid &g, const FWave &w, const KModel &m,
+                                                                                        const FStep &a, const EPlan &pl, const uint3 bid) {
     constexpr int W = kEdgeW, TZ = W - 4 * N, Q = 2 * N + 1;
     constexpr int NS = IMG ? 21 : 16; // prefetched per-cell operands at the update column
     static_assert(TZ > 0, "edge block too small for the halo");
     __shared__ float lv[2][2][W];        // [column parity][vx, vz][lane]
     __shared__ float ls[N + 1][7][W];    // [column mod N+1][theta, omega, gzx, gxx, F3, F0, q][lane]
     const int l = threadIdx.x;
-    const int b0 = blockIdx.x;
+    const int b0 = bid.x;
     const ERect R = b0 >= pl.r[3].first ? pl.r[3] : b0 >= pl.r[2].first ? pl.r[2] : b0 >= pl.r[1].first ? pl.r[1]
                                                                                                        : pl.r[0];
     const int lb = b0 - R.first, zb = lb % R.nzb, xb = lb / R.nzb;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     const bool z_st = l >= N && l < W - N && z >= N && z < g.Zn - N;
     const bool z_vin = z >= g.vlo && z < g.Zn - g.vlo;
     const bool z_inj = a.inj && z == a.zr;
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const float *c = g.c;
     const int wb = 1 - a.rb;
     const unsigned P4 = (unsigned)g.P * 4u, zb4 = (unsigned)z * 4u;
@@ -78,11 +78,11 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     int src_i = -1, src_j = -1;
     float src_v = 0.f;
     if (a.src) {
-        src_i = a.src[2 * blockIdx.z];
-        src_j = a.src[2 * blockIdx.z + 1];
-        src_v = a.wav[(long)blockIdx.z * g.nt + a.t];
+        src_i = a.src[2 * bid.z];
+        src_j = a.src[2 * bid.z + 1];
+        src_v = a.wav[(long)bid.z * g.nt + a.t];
     }
-    const bool img_on = IMG && !(a.t0 && a.t < a.t0[blockIdx.z]);
+    const bool img_on = IMG && !(a.t0 && a.t < a.t0[bid.z]);
     const bool z_int = z >= g.L && z < g.Zn - g.L;
     const float2 *const sp = img_on ? a.sp + so : nullptr;
     float *const npp = img_on ? a.npp + so : nullptr, *const nps = img_on ? a.nps + so : nullptr;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
         vx = v.x;
         vz = v.y;
         if (z_inj && i >= g.L && i < g.Xn - g.L) { // load_record, P-S-seperating.cu:1471-1483
-            long io = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
+            long io = ((long)bid.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
             vx = a.inj[io];
             vz = a.inj[io + (long)g.nt * g.nx];
         }
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
             const float ez = uo[5] + dt2 * (hz + Kz); const float sz = uo[3] + ez;
             stb2(DSp, k2, make_float2(ex, ez));
             stb2(Sp, k2, make_float2(sx, sz));
-            record(g, a, cu, z, vx, vz, sx, sz);
+            record(g, a, bid.z, cu, z, vx, vz, sx, sz);
             if (IMG && img_on && z_int && cu >= g.L && cu < g.Xn - g.L) { // corr_v, P-S-seperating.cu:1426-1439
                 const float sa = uo[16 % NS], sb = uo[17 % NS];
                 const float rpx = vx - sx, rpz = vz - sz;
@@ -315,6 +315,11 @@ __global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) voi
     } else {
         for (int cc = cbeg; cc < cend; ++cc) body(std::integral_constant<int, 0>{}, cc);
     }
+}
+template <int N, bool IMG>
+__global__ __launch_bounds__(kEdgeW) __attribute__((amdgpu_waves_per_eu(2))) void k_edge(FGrid g, FWave w, KModel m,
+                                                                                        FStep a, EPlan pl) {
+    edge_body<N, IMG>(g, w, m, a, pl, blockIdx3());
 }
 
 } // namespace ouc

@@ -589,6 +589,13 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in, floa
     });
 }
 
+ouc_status ouc_rtm_grouped(ouc_ctx *const *ctxs, int32_t nctx, const ouc_shot *const *shots,
+                           const int32_t *nshots, float *const *stack_pp_lap, float *const *stack_ps_lap) {
+    if (!ctxs || nctx < 1 || !ctxs[0]) return fail(nullptr, OUC_INVALID_ARG, "need at least one context");
+    if (!shots || !nshots) return fail(ctxs[0], OUC_INVALID_ARG, "shots and nshots must not be NULL");
+    return guarded(ctxs[0], [&] { fused_rtm_group(ctxs, nctx, shots, nshots, stack_pp_lap, stack_ps_lap); });
+}
+
 ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, float *stack_pp_lap,
                          float *stack_ps_lap, float *per_shot_ms) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");

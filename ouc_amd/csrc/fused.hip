@@ -17,6 +17,7 @@
 #include "edge_kernels.hpp"
 #include "fused_kernels.hpp"
 #include "fwi_kernels.hpp"
+#include "group_kernels.hpp"
 #include "lapfft.hpp"
 #include "lpad_kernels.hpp"
 
@@ -65,6 +66,9 @@ void launch_core(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &
 }
 
 constexpr int kWaveArrays = 32; // V[2][2], DV[2], S[2], DS[2], 6 stress, 4 F, 8 O, 4 F2
+
+struct GroupGraphs; // captured loops of a grouped run, kept by its first context
+void drop_group(GroupGraphs *g);
 
 struct FusedState {
     int cap = 0;           // shots the buffers hold
@@ -117,6 +121,7 @@ struct FusedState {
         }
     } gfwd, gbwd;
     void drop_graphs() { gfwd.drop(); gbwd.drop(); }
+    GroupGraphs *grp = nullptr;
     // side stream (default priority) for the reconstruction, which runs
     // beside the receiver step of the same backward iteration; fork/join events
     hipStream_t s2 = nullptr;
@@ -129,6 +134,7 @@ struct FusedState {
     }
     ~FusedState() {
         drop_graphs();
+        drop_group(grp);
         if (evf) (void)hipEventDestroy(evf);
         if (evj) (void)hipEventDestroy(evj);
         if (s2) (void)hipStreamDestroy(s2);
@@ -559,15 +565,24 @@ struct FwiBatch {
     int rblocks;          // residual blocks per shot (partial sums in st->part)
 };
 
-void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe,
-               const Snap *snap = nullptr, const float *rec_in = nullptr, const Snap *rsnap = nullptr,
-               const FwiBatch *fwi = nullptr) {
+// A batch's per-shot inputs on the device and its source state zeroed: what
+// the time loops of run_batch and of a grouped run start from.  `mute`: apply
+// the shots' direct-wave mute to the records.
+struct BatchIn {
+    int B = 0, zr = -1, rblocks = 0;
+    bool exact = false;
+    Launch L{};
+    KRing kr{};
+    KStrips ks0{};
+    KStripStride sstride{};
+    unsigned int *mon_f = nullptr, *mon_r = nullptr;
+    size_t span = 0; // bytes of one [shot][plane] array of the batch
+};
+
+BatchIn batch_begin(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, bool mute) {
     FusedState *st = fs(c);
     const Grid &g = c->g;
-    const FGrid &f = st->g;
     hipStream_t s = c->stream;
-    const Launch L = launches(st, B);
-    const bool exact = c->p.semantics == OUC_SEM_EXACT;
     const int nt = g.nt;
     // per-shot inputs
     std::vector<int> pos(2 * B), start(B, 0);
@@ -587,7 +602,7 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
         std::copy(w.begin(), w.end(), wavs.begin() + (size_t)b * nt + start[b]);
         if (zr < 0) zr = sh.rec_z + g.L;
         if (sh.rec_z + g.L != zr) throw Error(OUC_INVALID_ARG, "all shots of a batch must share the receiver row");
-        if (sh.mute && !fwi) {
+        if (sh.mute && mute) {
             std::vector<int32_t> cnt;
             mute_counts(g, sh, c->vp_pad[(size_t)(sh.sx + g.L) * g.P + zr], cnt);
             for (int ix = 0; ix < g.nx; ++ix) counts[(size_t)b * g.nx + ix] = std::min(nt, cnt[ix] + start[b]);
@@ -606,6 +621,55 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     const KStrips ks0 = base_strips(st, c);
     const KStripStride sstride{st->shot_strip, st->shot_chunks, st->shot_key, st->shot_aux, st->shot_in32};
     const int rblocks = (int)((c->ring.ncells + 255) / 256);
+    OUC_HIP(hipStreamSynchronize(s)); // the host arrays above go out of scope
+    BatchIn bi;
+    bi.B = B;
+    bi.zr = zr;
+    bi.rblocks = rblocks;
+    bi.exact = c->p.semantics == OUC_SEM_EXACT;
+    bi.L = launches(st, B);
+    bi.kr = kr;
+    bi.ks0 = ks0;
+    bi.sstride = sstride;
+    bi.mon_f = mon_f;
+    bi.mon_r = mon_r;
+    bi.span = span;
+    return bi;
+}
+
+// After the forward loop: direct-wave mute of the records, hand-off of the
+// source state (V^nt in buffer nt mod 2; V^{nt-1} into the other; source P
+// levels nt and nt-1), receiver state and imaging accumulators zeroed.
+void batch_handoff(ouc_ctx *c, const BatchIn &bi) {
+    FusedState *st = fs(c);
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const int nt = g.nt, rb = nt & 1;
+    {
+        dim3 gm((g.nx + 255) / 256, nt, bi.B);
+        k_fmute<<<gm, 256, 0, s>>>(g.nx, nt, st->counts.as<int32_t>(), st->rec.f());
+    }
+    k_funstep<<<bi.L.full, dim3(64, 4), 0, s>>>(st->g, st->S, rb, st->vp[nt % 3].as<float2>(),
+                                                st->vp[(nt - 1) % 3].as<float2>());
+    OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
+    for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, bi.span, s));
+}
+
+void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, CoreProbe &probe,
+               const Snap *snap = nullptr, const float *rec_in = nullptr, const Snap *rsnap = nullptr,
+               const FwiBatch *fwi = nullptr) {
+    FusedState *st = fs(c);
+    const Grid &g = c->g;
+    const FGrid &f = st->g;
+    hipStream_t s = c->stream;
+    const BatchIn bi = batch_begin(c, shots, B, rtm, !fwi);
+    const Launch &L = bi.L;
+    const bool exact = bi.exact;
+    const int nt = g.nt, zr = bi.zr, rblocks = bi.rblocks;
+    unsigned int *mon_f = bi.mon_f, *mon_r = bi.mon_r;
+    const KRing kr = bi.kr;
+    const KStrips ks0 = bi.ks0;
+    const KStripStride sstride = bi.sstride;
     int nlaunch = 0;
 
     // forward
@@ -670,16 +734,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     }
     c->last.forward_ms = 0; // filled by the caller from events
     OUC_HIP(hipEventRecord(c->ev[1], s));
-    {
-        dim3 gm((g.nx + 255) / 256, nt, B);
-        k_fmute<<<gm, 256, 0, s>>>(g.nx, nt, st->counts.as<int32_t>(), st->rec.f());
-    }
-    // hand-off: V^nt in buffer rb; V^{nt-1} into 1-rb; source P levels nt, nt-1
-    k_funstep<<<L.full, dim3(64, 4), 0, s>>>(f, st->S, rb, st->vp[nt % 3].as<float2>(),
-                                             st->vp[(nt - 1) % 3].as<float2>());
+    batch_handoff(c, bi);
     int rbs = 1 - rb;
-    OUC_HIP(hipMemsetAsync(st->mem_r.ptr, 0, (size_t)kWaveArrays * st->plane * st->cap * sizeof(float), s));
-    for (auto &b : st->acc) OUC_HIP(hipMemsetAsync(b.ptr, 0, span, s));
     const size_t gspan = (size_t)st->plane * st->cap; // floats per gradient accumulator
     if (fwi) OUC_HIP(hipMemsetAsync(st->gacc.ptr, 0, 3 * gspan * sizeof(float), s));
     int krs = 0;
@@ -951,6 +1007,390 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
         throw Error(OUC_NUMERIC_FALLBACK, std::to_string(diverged.size()) +
                                               " shot(s) ran again with CPML after the learned boundary diverged");
     }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped runs: the batches of several contexts (grids of different sizes) in
+// shared launches, one per kernel and time step (group_kernels.hpp).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct GroupGraphs {
+    std::vector<unsigned char> key; // the context table and loop shape the graphs were captured for
+    DevBuf tab;                     // device copy of the context table
+    hipGraphExec_t fwd = nullptr, bwd = nullptr;
+    void drop() {
+        if (fwd) (void)hipGraphExecDestroy(fwd);
+        if (bwd) (void)hipGraphExecDestroy(bwd);
+        fwd = bwd = nullptr;
+    }
+    ~GroupGraphs() { drop(); }
+};
+void drop_group(GroupGraphs *g) { delete g; }
+
+// Half-orders with grouped kernels.
+#define OUC_GDISPATCH(n, ...)                                                               \
+    switch (n) {                                                                            \
+    case 5: { constexpr int NN = 5; __VA_ARGS__; } break;                                   \
+    case 6: { constexpr int NN = 6; __VA_ARGS__; } break;                                   \
+    default: throw Error(OUC_UNSUPPORTED, "grouped runs support half-orders 5 and 6");       \
+    }
+
+GCtx make_gctx(ouc_ctx *c, const BatchIn &bi, int T) {
+    FusedState *st = fs(c);
+    const Grid &g = c->g;
+    GCtx x;
+    std::memset(&x, 0, sizeof x);
+    x.g = st->g;
+    x.rg = bi.L.rg;
+    x.S = st->S;
+    x.R = st->R;
+    x.km = c->kmodel();
+    x.pq = c->pq.as<float2>();
+    x.p2 = c->p2.f();
+    x.q2 = c->q2.f();
+    // step templates, as run_batch sets them for an RTM batch
+    x.fa.src = st->srcpos.as<int>();
+    x.fa.wav = st->wav.f();
+    x.fa.zr = bi.zr;
+    x.fa.rec = st->rec.f();
+    x.fa.ncomp = 2;
+    x.ra.zr = bi.zr;
+    x.ra.inj = st->rec.f();
+    x.ra.npp = st->acc[0].f();
+    x.ra.nps = st->acc[1].f();
+    x.ra.den = st->acc[2].f();
+    x.ra.t0 = st->t0.as<int>();
+    x.sa.src = st->srcpos.as<int>();
+    x.sa.wav = st->wav.f();
+    x.ring = bi.kr;
+    x.ks0 = bi.ks0;
+    x.sstride = bi.sstride;
+    for (int q = 0; q < 3; ++q) x.vp[q] = st->vp[q].as<float2>();
+    for (int v = 0; v < 2; ++v) {
+        x.ep[v] = st->eplan[v];
+        x.ep[v].save_ring = v == 0 ? 1 : 0; // the source's forward steps save strips
+        x.eb[v] = st->eblocks[v];
+    }
+    x.tiles_s = st->tiles_s.as<int2>(); x.nts = st->ntiles_s;
+    x.tiles_v = st->tiles_v.as<int2>(); x.ntv = st->ntiles_v;
+    x.tiles_rs = st->tiles_rs.as<int2>(); x.ntrs = st->ntiles_rs;
+    x.tiles_rv = st->tiles_rv.as<int2>(); x.ntrv = st->ntiles_rv;
+    x.tiles_p = st->tiles_p.as<int2>(); x.ntp = st->ntiles_p;
+    x.tiles_m = st->tiles_m.as<int2>(); x.ntm = st->ntiles_m;
+    x.lbcells = st->lbcells.as<LbCell>();
+    x.nlbcells = st->nlbcells;
+    x.lbrb = (st->nlbcells + 255) / 256;
+    x.lbw = st->lb_have ? st->lbw.f() : nullptr;
+    x.lbd = st->lbd.f();
+    x.lbsc = st->g.dt2 * st->g.idx;
+    x.mon_f = bi.mon_f;
+    x.mon_r = bi.mon_r;
+    x.nt = g.nt;
+    x.B = bi.B;
+    x.start = T - g.nt;
+    x.exact = bi.exact ? 1 : 0;
+    x.edge = st->edge ? 1 : 0;
+    x.rblocks = bi.rblocks;
+    x.cbx = bi.L.has_core ? (int)bi.L.core.x : 0;
+    x.cby = (int)bi.L.core.y;
+    x.rbx = bi.L.has_recon_core ? (int)bi.L.recon_core.x : 0;
+    x.rby = (int)bi.L.recon_core.y;
+    return x;
+}
+
+// One batch per context (B[i] shots of context i) through the RTM path in
+// shared launches.  Afterwards each context holds its accumulators, records
+// and monitor as after run_batch.
+void run_group_batch(const std::vector<ouc_ctx *> &cs, const std::vector<const ouc_shot *> &shots,
+                     const std::vector<int> &B) {
+    const int n = (int)cs.size();
+    ouc_ctx *lead = cs[0];
+    FusedState *st0 = fs(lead);
+    hipStream_t s = lead->stream;
+    int T = 0;
+    for (ouc_ctx *c : cs) T = std::max(T, c->g.nt);
+    std::vector<BatchIn> bi(n);
+    std::vector<GCtx> hc(n);
+    for (int i = 0; i < n; ++i) {
+        bi[i] = batch_begin(cs[i], shots[i], B[i], true, true); // waits for its context's stream
+        hc[i] = make_gctx(cs[i], bi[i], T);
+    }
+    const int N = lead->g.N;
+    const bool learned = lead->g.learned, edge = st0->edge, fp16 = st0->fp16 != 0;
+    const int pf = st0->pf >= 2 ? 2 : 1; // forward core: the first context's tuned prefetch (1 or 2)
+    const bool graph = lead->graphs && !lead->profiling;
+    if (!st0->grp) st0->grp = new GroupGraphs();
+    GroupGraphs &G = *st0->grp;
+    std::vector<unsigned char> key(n * sizeof(GCtx) + 3 * sizeof(int));
+    std::memcpy(key.data(), hc.data(), n * sizeof(GCtx));
+    const int tail[3] = {T, pf, graph ? 1 : 0};
+    std::memcpy(key.data() + n * sizeof(GCtx), tail, sizeof tail);
+    if (G.key != key) {
+        G.drop();
+        G.tab.alloc(n * sizeof(GCtx));
+        copy_h2d(s, G.tab.ptr, hc.data(), n * sizeof(GCtx));
+        G.key = key;
+    }
+    const GCtx *tab = G.tab.as<GCtx>();
+    // launch over the contexts active at group step tau, blocks(i, t) blocks for context i at its step t
+    auto mk = [&](int tau, auto blocks, int &nb) {
+        GLaunch Lg;
+        std::memset(&Lg, 0, sizeof Lg);
+        Lg.tau = tau;
+        nb = 0;
+        for (int i = 0; i < n; ++i) {
+            const int t = tau - hc[i].start;
+            if (t < 0) continue;
+            const int k = blocks(i, t);
+            if (k <= 0) continue;
+            Lg.ctx[Lg.n] = i;
+            Lg.first[Lg.n] = nb;
+            nb += k;
+            ++Lg.n;
+        }
+        Lg.first[Lg.n] = nb;
+        return Lg;
+    };
+    // one wavefield step (wave_step_probed): rcv 0 the source's forward run, 1 the receiver run
+    auto wave_step = [&](int tau, auto rcvc, hipStream_t q) {
+        constexpr bool RCV = decltype(rcvc)::value;
+        constexpr int rcv = RCV ? 1 : 0;
+        int nb;
+        GLaunch Lg;
+        if (learned) {
+            Lg = mk(tau, [&](int i, int) { return hc[i].ntp * hc[i].B; }, nb);
+            if (nb) k_lpad_stress_grp<RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg);
+        }
+        if (edge) {
+            Lg = mk(tau, [&](int i, int) { return hc[i].eb[rcv] * hc[i].B; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_edge_grp<NN, RCV><<<nb, kEdgeW, 0, q>>>(tab, Lg))
+        } else {
+            Lg = mk(tau, [&](int i, int) { return hc[i].nts * hc[i].B; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_band_stress_grp<NN, RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg))
+        }
+        Lg = mk(tau, [&](int i, int) { return hc[i].cbx * hc[i].cby * hc[i].B; }, nb);
+        if (nb) {
+            if (RCV) OUC_GDISPATCH(N, k_core_grp<NN, CORE_IMAGE, 1><<<nb, 256, 0, q>>>(tab, Lg))
+            else if (pf == 2) OUC_GDISPATCH(N, k_core_grp<NN, CORE_STEP, 2><<<nb, 256, 0, q>>>(tab, Lg))
+            else OUC_GDISPATCH(N, k_core_grp<NN, CORE_STEP, 1><<<nb, 256, 0, q>>>(tab, Lg))
+        }
+        if (!edge) {
+            Lg = mk(tau, [&](int i, int) { return hc[i].ntv * hc[i].B; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_band_velocity_grp<NN, RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg))
+        }
+        if (learned) {
+            Lg = mk(tau, [&](int i, int) { return hc[i].lbrb * hc[i].B; }, nb);
+            if (nb) k_lb_ring_grp<RCV><<<nb, 256, 0, q>>>(tab, Lg);
+            Lg = mk(tau, [&](int i, int) { return hc[i].lbw ? hc[i].lbrb * hc[i].B : 0; }, nb);
+            if (nb) k_lb_apply_grp<RCV><<<nb, 256, 0, q>>>(tab, Lg);
+            Lg = mk(tau, [&](int i, int) { return hc[i].ntp * hc[i].B; }, nb);
+            if (nb) k_lpad_velocity_grp<RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg);
+            Lg = mk(tau, [&](int i, int t) { return t % kMonStride == 0 ? hc[i].ntm * hc[i].B : 0; }, nb);
+            if (nb) k_lb_monitor_grp<RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg);
+        }
+    };
+    auto run = [&](hipGraphExec_t &ex, auto &&body) {
+        if (!graph) {
+            body();
+            return;
+        }
+        if (!ex) {
+            hipGraph_t gr = nullptr;
+            OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            OUC_HIP(hipStreamEndCapture(s, &gr));
+            hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            OUC_HIP(e);
+        }
+        OUC_HIP(hipGraphLaunch(ex, s));
+    };
+    OUC_HIP(hipEventRecord(lead->ev[4], s));
+    run(G.fwd, [&] {
+        for (int tau = 0; tau < T; ++tau) {
+            wave_step(tau, std::false_type{}, s);
+            int nb;
+            const GLaunch Lg = mk(tau, [&](int i, int) { return hc[i].rblocks * hc[i].B; }, nb);
+            if (nb) k_fstrip_save_grp<<<nb, 256, 0, s>>>(tab, Lg);
+        }
+        OUC_HIP(hipGetLastError());
+    });
+    OUC_HIP(hipEventRecord(lead->ev[1], s));
+    OUC_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) {
+        batch_handoff(cs[i], bi[i]);
+        OUC_HIP(hipStreamSynchronize(cs[i]->stream));
+    }
+    // backward: the reconstruction on a second stream beside the receiver step
+    // (captured loops only), joined every step, as in run_batch
+    if (graph) st0->ensure_side();
+    hipStream_t sr = graph ? st0->s2 : s;
+    OUC_HIP(hipEventRecord(lead->ev[6], s));
+    run(G.bwd, [&] {
+        for (int tau = T - 1; tau >= 0; --tau) {
+            if (graph) {
+                OUC_HIP(hipEventRecord(st0->evf, s));
+                OUC_HIP(hipStreamWaitEvent(sr, st0->evf, 0));
+            }
+            int nb;
+            GLaunch Lg;
+            if (fp16) {
+                Lg = mk(tau, [&](int i, int) { return hc[i].rblocks * RS_NF * hc[i].B; }, nb);
+                if (nb) k_strip_decode_grp<<<nb, 256, 0, sr>>>(tab, Lg);
+            }
+            Lg = mk(tau, [&](int i, int t) { return t < hc[i].nt - 1 ? hc[i].ntrs * hc[i].B : 0; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_recon_stress_grp<NN><<<nb, dim3(64, 4), 0, sr>>>(tab, Lg))
+            Lg = mk(tau, [&](int i, int t) { return t < hc[i].nt - 1 ? hc[i].rbx * hc[i].rby * hc[i].B : 0; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_core_grp<NN, CORE_RECON, 2><<<nb, 256, 0, sr>>>(tab, Lg))
+            Lg = mk(tau, [&](int i, int t) { return t < hc[i].nt - 1 ? hc[i].ntrv * hc[i].B : 0; }, nb);
+            if (nb) OUC_GDISPATCH(N, k_recon_velocity_grp<NN><<<nb, dim3(64, 4), 0, sr>>>(tab, Lg))
+            wave_step(tau, std::true_type{}, s);
+            if (graph) {
+                OUC_HIP(hipEventRecord(st0->evj, sr));
+                OUC_HIP(hipStreamWaitEvent(s, st0->evj, 0));
+            }
+        }
+        OUC_HIP(hipGetLastError());
+    });
+    OUC_HIP(hipEventRecord(lead->ev[2], s));
+    OUC_HIP(hipStreamSynchronize(s));
+}
+
+// Capacities of the contexts of a grouped run: each context's shots, or its
+// set batch capacity, scaled down together when all do not fit in 85 % of the
+// device memory free or held by them.
+std::vector<int> group_caps(const std::vector<ouc_ctx *> &cs, const std::vector<int> &nshots) {
+    size_t freeb = 0, total = 0;
+    OUC_HIP(hipMemGetInfo(&freeb, &total));
+    size_t held = 0;
+    for (ouc_ctx *c : cs) {
+        FusedState *st = fs(c);
+        held += st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->in32.bytes +
+                st->e8.bytes + st->keyb.bytes + st->ef.bytes + st->decb.bytes + st->gacc.bytes + st->obs.bytes;
+    }
+    const double avail = (double)(freeb + held) * 0.85;
+    std::vector<int> want(cs.size());
+    double need = 0;
+    for (size_t i = 0; i < cs.size(); ++i) {
+        want[i] = nshots[i];
+        if (cs[i]->batch_cap > 0) want[i] = std::min(want[i], cs[i]->batch_cap);
+        need += (double)want[i] * per_shot_bytes(cs[i], cs[i]->p.strip_precision == OUC_STRIP_FP16);
+    }
+    if (need > avail)
+        for (auto &w : want) w = std::max(1, (int)(w * (avail / need)));
+    return want;
+}
+
+} // namespace
+
+void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, const int32_t *nshots,
+                     float *const *stack_pp, float *const *stack_ps) {
+    if (n < 1 || n > kGroupMax) throw Error(OUC_INVALID_ARG, "a group holds 1 to " + std::to_string(kGroupMax) + " contexts");
+    std::vector<ouc_ctx *> cs(ctxs, ctxs + n);
+    const ouc_ctx *a = cs[0];
+    for (int i = 0; i < n; ++i) {
+        const ouc_ctx *c = cs[i];
+        if (!c) throw Error(OUC_INVALID_ARG, "context " + std::to_string(i) + " is NULL");
+        for (int j = 0; j < i; ++j)
+            if (cs[j] == c) throw Error(OUC_INVALID_ARG, "a context appears twice in the group");
+        if (c->p.engine != OUC_ENGINE_FUSED) throw Error(OUC_UNSUPPORTED, "grouped runs need the fused engine");
+        if (c->device != a->device || c->g.N != a->g.N || c->g.learned != a->g.learned ||
+            c->p.strip_precision != a->p.strip_precision || c->p.semantics != a->p.semantics)
+            throw Error(OUC_INVALID_ARG,
+                        "contexts of a group must share the device, half-order, boundary kind, strip precision and semantics");
+        if (nshots[i] < 0 || (nshots[i] > 0 && !shots[i])) throw Error(OUC_INVALID_ARG, "bad shot list");
+        const Grid &g = c->g;
+        for (int k = 0; k < nshots[i]; ++k) {
+            const ouc_shot *sh = &shots[i][k];
+            if (sh->sx < 0 || sh->sx >= g.nx || sh->sz < 0 || sh->sz >= g.nz || sh->rec_z < 0 || sh->rec_z >= g.nz)
+                throw Error(OUC_INVALID_ARG, "shot " + std::to_string(k) + " of context " + std::to_string(i) +
+                                                 " outside the interior");
+            if (!sh->wavelet && !(sh->fm > 0)) throw Error(OUC_INVALID_ARG, "fm must be positive");
+            if (sh->nt > g.nt) throw Error(OUC_INVALID_ARG, "shot nt beyond its context's");
+        }
+    }
+    if (a->g.N != 5 && a->g.N != 6) throw Error(OUC_UNSUPPORTED, "grouped runs support half-orders 5 and 6");
+    const std::vector<int> cap = group_caps(cs, std::vector<int>(nshots, nshots + n));
+    for (int i = 0; i < n; ++i) ensure(cs[i], std::max(1, cap[i]));
+    std::vector<DevBuf> stack(n);
+    std::vector<std::vector<int>> diverged(n);
+    for (int i = 0; i < n; ++i) {
+        const size_t ni = (size_t)cs[i]->g.nx * cs[i]->g.nz;
+        stack[i].alloc(2 * ni * sizeof(float));
+        OUC_HIP(hipMemsetAsync(stack[i].ptr, 0, 2 * ni * sizeof(float), cs[i]->stream));
+        cs[i]->last = ouc_timings{};
+    }
+    std::vector<int> off(n, 0);
+    double fwd_ms = 0, bwd_ms = 0, tot_ms = 0;
+    for (;;) {
+        std::vector<ouc_ctx *> rc;
+        std::vector<const ouc_shot *> rs;
+        std::vector<int> rb, ri;
+        for (int i = 0; i < n; ++i)
+            if (off[i] < nshots[i]) {
+                rc.push_back(cs[i]);
+                rs.push_back(shots[i] + off[i]);
+                rb.push_back(std::min(cap[i], nshots[i] - off[i]));
+                ri.push_back(i);
+            }
+        if (rc.empty()) break;
+        run_group_batch(rc, rs, rb);
+        float f = 0, b = 0, w = 0;
+        OUC_HIP(hipEventElapsedTime(&f, rc[0]->ev[4], rc[0]->ev[1]));
+        OUC_HIP(hipEventElapsedTime(&b, rc[0]->ev[6], rc[0]->ev[2]));
+        OUC_HIP(hipEventElapsedTime(&w, rc[0]->ev[4], rc[0]->ev[2]));
+        fwd_ms += f;
+        bwd_ms += b;
+        tot_ms += w;
+        for (size_t k = 0; k < rc.size(); ++k) {
+            ouc_ctx *c = rc[k];
+            const int i = ri[k];
+            FusedState *st = fs(c);
+            const size_t ni = (size_t)c->g.nx * c->g.nz;
+            std::vector<char> bad = monitor_verdict(c, rb[k]);
+            for (int b2 = 0; b2 < rb[k]; ++b2) {
+                if (bad[b2]) {
+                    diverged[i].push_back(off[i] + b2);
+                    continue;
+                }
+                shot_images(c, st, b2);
+                k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[2].f(), stack[i].f());
+                k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[3].f(),
+                                                                                  stack[i].f() + ni);
+            }
+            OUC_HIP(hipStreamSynchronize(c->stream));
+            off[i] += rb[k];
+        }
+    }
+    int nfb = 0;
+    for (int i = 0; i < n; ++i) {
+        ouc_ctx *c = cs[i];
+        const Grid &g = c->g;
+        const size_t ni = (size_t)g.nx * g.nz;
+        if (stack_pp && stack_pp[i]) copy_d2h(c->stream, stack_pp[i], stack[i].ptr, ni * sizeof(float));
+        if (stack_ps && stack_ps[i]) copy_d2h(c->stream, stack_ps[i], stack[i].f() + ni, ni * sizeof(float));
+        double fb_ms = 0;
+        if (!diverged[i].empty()) {
+            fb_ms = rerun_with_cpml(c, shots[i], diverged[i], stack_pp ? stack_pp[i] : nullptr,
+                                    stack_ps ? stack_ps[i] : nullptr, nullptr, tot_ms);
+            nfb += (int)diverged[i].size();
+            c->last.fallback_shots = (int32_t)diverged[i].size();
+        }
+        c->last.total_ms = tot_ms + fb_ms;
+        c->last.forward_ms = fwd_ms;
+        c->last.backward_ms = bwd_ms;
+        const double padded = (double)g.Xn * g.Zn, interior = (double)g.nx * g.nz;
+        c->last.cell_updates = (double)nshots[i] * g.nt * (2 * padded + interior);
+    }
+    if (nfb)
+        throw Error(OUC_NUMERIC_FALLBACK, std::to_string(nfb) + " shot(s) ran again with CPML after the learned boundary diverged");
 }
 
 // Single shot through the fused path (a batch of one), returning its images.

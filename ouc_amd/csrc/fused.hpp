@@ -14,6 +14,8 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *im
                    float *lap_ps);
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms);
+void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, const int32_t *nshots,
+                     float *const *stack_pp, float *const *stack_ps);
 double fused_own_bytes(const ouc_ctx *c);
 // FWI gradient (gradfun with the adsource residual) summed over shots.
 void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const float *observed, float *grad_vp,

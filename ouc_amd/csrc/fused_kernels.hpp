@@ -42,6 +42,11 @@ __device__ __forceinline__ void static_for(F &&f) {
     static_for_seq(f, std::make_integer_sequence<int, Q>{});
 }
 
+// Kernel bodies take the block index as an argument: a plain launch passes its
+// own, a grouped launch (several grids in one launch) the block's index within
+// its context.
+__device__ __forceinline__ uint3 blockIdx3() { return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z); }
+
 // Geometry shared by the fused kernels.
 struct FGrid {
     int Xn, Zn, P, L, N, nx, nz, nt;
@@ -106,7 +111,7 @@ __device__ __forceinline__ bool in_update(const FGrid &g, int i, int j) {
 // V as seen by the stress pass at (i, j): the state, zero outside the update
 // region, the injected record on the receiver row (load_record,
 // P-S-seperating.cu:1471-1483).
-__device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const FStep &a, long so, int i, int j,
+__device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const FStep &a, int shot, long so, int i, int j,
                                          float &vx, float &vz) {
     vx = 0.f;
     vz = 0.f;
@@ -116,24 +121,24 @@ __device__ __forceinline__ void load_vin(const FGrid &g, const FWave &w, const F
     vx = v.x;
     vz = v.y;
     if (a.inj && j == a.zr && i >= g.L && i < g.Xn - g.L) {
-        long o = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
+        long o = ((long)shot * 2 * g.nt + a.t) * g.nx + (i - g.L);
         vx = a.inj[o];
         vz = a.inj[o + (long)g.nt * g.nx];
     }
 }
 
-__device__ __forceinline__ float src_term(const FGrid &g, const FStep &a, int i, int j) {
+__device__ __forceinline__ float src_term(const FGrid &g, const FStep &a, int shot, int i, int j) {
     if (!a.src) return 0.f;
-    const int s = blockIdx.z;
+    const int s = shot;
     return (i == a.src[2 * s] && j == a.src[2 * s + 1]) ? a.wav[(long)s * g.nt + a.t] : 0.f;
 }
 
 // Record totals and P/S parts after the update (P-S-seperating.cu:307-311).
-__device__ __forceinline__ void record(const FGrid &g, const FStep &a, int i, int j, float vx, float vz, float sx,
+__device__ __forceinline__ void record(const FGrid &g, const FStep &a, int shot, int i, int j, float vx, float vz, float sx,
                                        float sz) {
     if (!a.rec || j != a.zr || i < g.L || i >= g.Xn - g.L) return;
     const long plane = (long)g.nt * g.nx;
-    long o = ((long)blockIdx.z * a.ncomp) * plane + (long)a.t * g.nx + (i - g.L);
+    long o = ((long)shot * a.ncomp) * plane + (long)a.t * g.nx + (i - g.L);
     a.rec[o] = vx;
     a.rec[o + plane] = vz;
     if (a.ncomp == 6) {
@@ -145,10 +150,10 @@ __device__ __forceinline__ void record(const FGrid &g, const FStep &a, int i, in
 }
 
 // corr_v (P-S-seperating.cu:1426-1439) for an interior cell.
-__device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, long k, int i, int j, float vx, float vz,
+__device__ __forceinline__ void correlate(const FGrid &g, const FStep &a, int shot, long k, int i, int j, float vx, float vz,
                                           float sx, float sz) {
     if (!a.sp || i < g.L || i >= g.Xn - g.L || j < g.L || j >= g.Zn - g.L) return;
-    if (a.t0 && a.t < a.t0[blockIdx.z]) return;
+    if (a.t0 && a.t < a.t0[shot]) return;
     const float2 spk = a.sp[k];
     float sa = spk.x, sb = spk.y;
     float rpx = vx - sx, rpz = vz - sz;
@@ -175,7 +180,7 @@ struct FRecon {
     float2 *vp;          // [shot][plane] output P level (x, z), or NULL (no imaging condition)
 };
 
-__device__ __forceinline__ KStrips shot_strips(const FRecon &rc) { return strips_of_shot(rc.strips, rc.stride, blockIdx.z); }
+__device__ __forceinline__ KStrips shot_strips(const FRecon &rc, int shot) { return strips_of_shot(rc.strips, rc.stride, shot); }
 
 // Totals seen by the reconstruction's stress pass at (i, j).
 __device__ __forceinline__ void recon_vin(const FGrid &g, const FWave &w, const FRecon &rc, const KStrips &s, int rb,
@@ -233,8 +238,8 @@ __device__ __forceinline__ void stb2(float2 *base, unsigned boff, float2 v) {
 // off the ring.
 enum { CORE_STEP = 0, CORE_RECON = 1, CORE_IMAGE = 2 };
 template <int N, int MODE, int PF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float2 *__restrict__ pq,
-                                              FStep a, FRecon rc) {
+__device__ __forceinline__ void core_body(const FGrid &g, const FWave &w, const float2 *__restrict__ pq,
+                                              const FStep &a, const FRecon &rc, const uint3 bid) {
     constexpr bool RECON = MODE == CORE_RECON, IMG = MODE == CORE_IMAGE;
     constexpr int NS = IMG ? 11 : 6; // prefetched per-cell operands: state (+ imaging)
     constexpr int TZ = kCoreTZ;
@@ -243,10 +248,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ float lv[2][2][256]; // [column parity][vx, vz][lane]
     __shared__ float ls[2][5][256]; // [column parity][theta, omega, gzx, gxx, q][lane]
     const int l = threadIdx.x;
-    const int z = core_zalign(g.cz0) + blockIdx.y * TZ - 2 * N + l;
-    const int xs = g.cx0 + blockIdx.x * g.csx;
+    const int z = core_zalign(g.cz0) + bid.y * TZ - 2 * N + l;
+    const int xs = g.cx0 + bid.x * g.csx;
     const int xe = min(xs + g.csx, g.cx1);
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const float *c = g.c;
     const int wb = 1 - a.rb;
     const bool z_out = z >= g.cz0 && z < g.cz1 && l >= 2 * N && l < 2 * N + TZ;
@@ -264,17 +269,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     int src_i = -1, src_j = -1;
     float src_v = 0.f;
     if (a.src && (!RECON || rc.exact)) {
-        src_i = a.src[2 * blockIdx.z];
-        src_j = a.src[2 * blockIdx.z + 1];
-        src_v = a.wav[(long)blockIdx.z * g.nt + a.t];
+        src_i = a.src[2 * bid.z];
+        src_j = a.src[2 * bid.z + 1];
+        src_v = a.wav[(long)bid.z * g.nt + a.t];
     }
     // imaging condition on this step (shots that start late image only their own steps)
-    const bool img_on = IMG && !(a.t0 && a.t < a.t0[blockIdx.z]);
+    const bool img_on = IMG && !(a.t0 && a.t < a.t0[bid.z]);
     const float2 *const sp = img_on ? a.sp + so : nullptr;
     float *const npp = img_on ? a.npp + so : nullptr, *const nps = img_on ? a.nps + so : nullptr;
     float *const den = img_on ? a.den + so : nullptr;
     KStrips rs{};
-    if (RECON) rs = shot_strips(rc);
+    if (RECON) rs = shot_strips(rc, bid.z);
     auto vin = [&](int i, float &vx, float &vz) {
         if (RECON) {
             recon_vin(g, w, rc, rs, a.rb, so, i, z, vx, vz);
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         vx = v.x;
         vz = v.y;
         if (z_inj && i >= g.L && i < g.Xn - g.L) { // load_record, P-S-seperating.cu:1471-1483
-            long io = ((long)blockIdx.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
+            long io = ((long)bid.z * 2 * g.nt + a.t) * g.nx + (i - g.L);
             vx = a.inj[io];
             vz = a.inj[io + (long)g.nt * g.nx];
         }
@@ -447,7 +452,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (rc.vp) // the source P level for the imaging condition (none for the FWI gradient)
                     stb2(rc.vp + so, k, make_float2(vx - sx, vz - sz));
             } else {
-                record(g, a, cu, z, vx, vz, sx, sz);
+                record(g, a, bid.z, cu, z, vx, vz, sx, sz);
                 if (IMG && img_on) { // corr_v (P-S-seperating.cu:1426-1439); core cells are interior
                     const float sa = st6[6 % NS], sb = st6[7 % NS];
                     const float rpx = vx - sx, rpz = vz - sz;
@@ -470,6 +475,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int cc = cbeg; cc < cend; ++cc) body(std::integral_constant<int, 0>{}, cc);
     }
 }
+template <int N, int MODE, int PF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float2 *__restrict__ pq,
+                                              FStep a, FRecon rc) {
+    core_body<N, MODE, PF>(g, w, pq, a, rc, blockIdx3());
+}
 
 // ---------------------------------------------------------------------------
 // Band kernels: 64 x 4 tiles from a host-built list (tile origin i0, j0).
@@ -480,31 +490,31 @@ __device__ __forceinline__ bool in_core(const FGrid &g, int i, int j) {
 
 // Stress with CPML on band cells and the core's outer N-cell margin.
 template <int N>
-__global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
-                                                     FStep a) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void band_stress_body(const FGrid &g, const FWave &w, const KModel &m, const int2 *__restrict__ tiles,
+                                                     const FStep &a, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     if (!in_update(g, i, j) || in_lpad(g, i, j)) return;
     // skip core cells farther than N from the band
     if (i >= g.cx0 + N && i < g.cx1 - N && j >= g.cz0 + N && j < g.cz1 - N) return;
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const float *c = g.c;
     float s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
 #pragma unroll
     for (int mm = 1; mm <= N; ++mm) {
         float a0, b0, a1, b1;
-        load_vin(g, w, a, so, i + mm - 1, j, a0, b0);
-        load_vin(g, w, a, so, i - mm, j, a1, b1);
+        load_vin(g, w, a, bid.z, so, i + mm - 1, j, a0, b0);
+        load_vin(g, w, a, bid.z, so, i - mm, j, a1, b1);
         s1 += c[mm - 1] * (a0 - a1); // D-x Vx
-        load_vin(g, w, a, so, i + mm, j, a0, b0);
-        load_vin(g, w, a, so, i - mm + 1, j, a1, b1);
+        load_vin(g, w, a, bid.z, so, i + mm, j, a0, b0);
+        load_vin(g, w, a, bid.z, so, i - mm + 1, j, a1, b1);
         s2 += c[mm - 1] * (b0 - b1); // D+x Vz
-        load_vin(g, w, a, so, i, j + mm, a0, b0);
-        load_vin(g, w, a, so, i, j - mm + 1, a1, b1);
+        load_vin(g, w, a, bid.z, so, i, j + mm, a0, b0);
+        load_vin(g, w, a, bid.z, so, i, j - mm + 1, a1, b1);
         s3 += c[mm - 1] * (a0 - a1); // D+z Vx
-        load_vin(g, w, a, so, i, j + mm - 1, a0, b0);
-        load_vin(g, w, a, so, i, j - mm, a1, b1);
+        load_vin(g, w, a, bid.z, so, i, j + mm - 1, a0, b0);
+        load_vin(g, w, a, bid.z, so, i, j - mm, a1, b1);
         s4 += c[mm - 1] * (b0 - b1); // D-z Vz
     }
     const float gxx = s1 * g.idx, gzx = s2 * g.idx, gxz = s3 * g.idz, gzz = s4 * g.idz;
@@ -519,9 +529,14 @@ __global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m,
         Fzx = Fzx + (a1 * gzx - a1 * Fzx) * dt;
         w.F[0][k] = Fxx; w.F[1][k] = Fzz; w.F[2][k] = Fxz; w.F[3][k] = Fzx;
     }
-    w.th[k] = m.p[k0] * (gxx + gzz - Fxx - Fzz) + src_term(g, a, i, j);
+    w.th[k] = m.p[k0] * (gxx + gzz - Fxx - Fzz) + src_term(g, a, bid.z, i, j);
     w.om[k] = m.q[k0] * (gxz - gzx - Fxz + Fzx);
     w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
+}
+template <int N>
+__global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                     FStep a) {
+    band_stress_body<N>(g, w, m, tiles, a, blockIdx3());
 }
 
 // Velocity with CPML on band cells.  Every array is addressed from a
@@ -529,12 +544,12 @@ __global__ __launch_bounds__(256) void k_band_stress(FGrid g, FWave w, KModel m,
 // the z taps are immediate offsets of one address and the x taps share one
 // address per column offset across all fields.
 template <int N>
-__global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
-                                                       FStep a) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void band_velocity_body(const FGrid &g, const FWave &w, const KModel &m, const int2 *__restrict__ tiles,
+                                                       const FStep &a, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     if (!in_update(g, i, j) || in_core(g, i, j) || in_lpad(g, i, j)) return;
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const unsigned o = (unsigned)k0 * 4u; // byte offset of the cell in a plane
     const int P4 = g.P * 4;
@@ -631,8 +646,13 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
     const float ez = ds.y + dt2 * (hz + Kz); const float sz = sv.y + ez;
     stb2(DSp, o2, make_float2(ex, ez));
     stb2(Sp, o2, make_float2(sx, sz));
-    record(g, a, i, j, vx, vz, sx, sz);
-    correlate(g, a, k, i, j, vx, vz, sx, sz);
+    record(g, a, bid.z, i, j, vx, vz, sx, sz);
+    correlate(g, a, bid.z, k, i, j, vx, vz, sx, sz);
+}
+template <int N>
+__global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                       FStep a) {
+    band_velocity_body<N>(g, w, m, tiles, a, blockIdx3());
 }
 
 // ---------------------------------------------------------------------------
@@ -647,14 +667,14 @@ __global__ __launch_bounds__(256) void k_band_velocity(FGrid g, FWave w, KModel 
 // strips) are staged in LDS once, so each total is looked up once per tile
 // rather than once per stencil tap.
 template <int N>
-__global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const float *__restrict__ p2,
-                                                        const float *__restrict__ q2, FRecon rc, FStep a,
-                                                        const int2 *__restrict__ tiles) {
+__device__ __forceinline__ void recon_stress_body(const FGrid &g, const FWave &w, const float *__restrict__ p2,
+                                                        const float *__restrict__ q2, const FRecon &rc, const FStep &a,
+                                                        const int2 *__restrict__ tiles, const uint3 bid) {
     constexpr int HX = 4 + 2 * N, HZ = 64 + 2 * N;
     __shared__ float lx[HX][HZ], lz[HX][HZ];
-    const int2 t0 = tiles[blockIdx.x];
-    const long so = (long)blockIdx.z * g.plane;
-    const KStrips s = shot_strips(rc);
+    const int2 t0 = tiles[bid.x];
+    const long so = (long)bid.z * g.plane;
+    const KStrips s = shot_strips(rc, bid.z);
     for (int e = threadIdx.y * 64 + threadIdx.x; e < HX * HZ; e += 256) {
         const int xi = e / HZ, zj = e - xi * HZ;
         float vx, vz;
@@ -693,23 +713,29 @@ __global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const 
         gxx = s1 * g.idx; gzx = s2 * g.idx; gxz = s3 * g.idz; gzz = s4 * g.idz;
         const long k0 = (long)i * g.P + j;
         th = p2[k0] * (gxx + gzz);
-        if (rc.exact) th += src_term(g, a, i, j);
+        if (rc.exact) th += src_term(g, a, bid.z, i, j);
         om = q2[k0] * (gxz - gzx);
     }
     const long k = so + (long)i * g.P + j;
     w.th[k] = th; w.om[k] = om; w.gxx[k] = gxx; w.gxz[k] = gxz; w.gzx[k] = gzx; w.gzz[k] = gzz;
 }
+template <int N>
+__global__ __launch_bounds__(256) void k_recon_stress_f(FGrid g, FWave w, const float *__restrict__ p2,
+                                                        const float *__restrict__ q2, FRecon rc, FStep a,
+                                                        const int2 *__restrict__ tiles) {
+    recon_stress_body<N>(g, w, p2, q2, rc, a, tiles, blockIdx3());
+}
 
 // Reverse velocity step on the interior from the stored stress-like fields.
 template <int N>
-__global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, const float *__restrict__ q2, FRecon rc,
-                                                          FStep a, const int2 *__restrict__ tiles) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void recon_velocity_body(const FGrid &g, const FWave &w, const float *__restrict__ q2, const FRecon &rc,
+                                                          const FStep &a, const int2 *__restrict__ tiles, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     const int L = g.L;
     if (i < L || j < L || i >= g.Xn - L || j >= g.Zn - L) return;
     if (i >= L + 2 * N && i < g.Xn - L - 2 * N && j >= L + 2 * N && j < g.Zn - L - 2 * N) return; // recon core
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const float *c = g.c;
     const int P = g.P;
@@ -744,17 +770,22 @@ __global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, cons
     w.S.p[k] = make_float2(sx, sz);
     if (rc.vp) rc.vp[k] = make_float2(vx - sx, vz - sz);
 }
+template <int N>
+__global__ __launch_bounds__(256) void k_recon_velocity_f(FGrid g, FWave w, const float *__restrict__ q2, FRecon rc,
+                                                          FStep a, const int2 *__restrict__ tiles) {
+    recon_velocity_body<N>(g, w, q2, rc, a, tiles, blockIdx3());
+}
 
 // Strip store from the stored band stress and the written V (forward):
 // with_v 0 stores the stress-like fields, 1 the velocities, 2 both.
-__global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, KStripStride d, int slot,
-                                                     int vbuf, int with_v) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fstrip_save_body(const FGrid &g, const FWave &w, const KRing &r, const KStrips &s0, const KStripStride &d, int slot,
+                                                     int vbuf, int with_v, const uint3 bid) {
+    long q = (long)bid.x * blockDim.x + threadIdx.x;
     bool valid = q < r.ncells;
     int i = r.L, j = r.L;
     if (valid) ring_cell(r, q, i, j);
-    const KStrips s = strips_of_shot(s0, d, blockIdx.z);
-    const long so = (long)blockIdx.z * g.plane;
+    const KStrips s = strips_of_shot(s0, d, bid.z);
+    const long so = (long)bid.z * g.plane;
     const long k = so + (long)i * g.P + j;
     if (with_v) {
         const float2 v = valid ? w.V[vbuf].p[k] : make_float2(0.f, 0.f);
@@ -770,14 +801,18 @@ __global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, 
         strip_put(s, slot, RS_GXZ, q, valid, valid ? w.gxz[k] - w.F[2][k] : 0.f);
     }
 }
+__global__ __launch_bounds__(256) void k_fstrip_save(FGrid g, FWave w, KRing r, KStrips s0, KStripStride d, int slot,
+                                                     int vbuf, int with_v) {
+    fstrip_save_body(g, w, r, s0, d, slot, vbuf, with_v, blockIdx3());
+}
 
 // Hand-off: (V^nt, D) -> (V^{nt-1}, D) into the other V buffer; P velocity
 // of levels nt and nt-1.
-__global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float2 *__restrict__ vp_nt,
-                                                 float2 *__restrict__ vp_m1) {
-    const int j = blockIdx.x * 64 + threadIdx.x, i = blockIdx.y * 4 + threadIdx.y;
+__device__ __forceinline__ void funstep_body(const FGrid &g, const FWave &w, int rb, float2 *__restrict__ vp_nt,
+                                                 float2 *__restrict__ vp_m1, const uint3 bid) {
+    const int j = bid.x * 64 + threadIdx.x, i = bid.y * 4 + threadIdx.y;
     if (i >= g.Xn || j >= g.Zn) return;
-    const long k = (long)blockIdx.z * g.plane + (long)i * g.P + j;
+    const long k = (long)bid.z * g.plane + (long)i * g.P + j;
     const float2 v = w.V[rb].p[k], sv = w.S.p[k], dv = w.DV.p[k], ds = w.DS.p[k];
     const float vx = v.x, vz = v.y, sx = sv.x, sz = sv.y;
     vp_nt[k] = make_float2(vx - sx, vz - sz);
@@ -785,6 +820,10 @@ __global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float
     w.V[1 - rb].p[k] = make_float2(vx1, vz1);
     w.S.p[k] = make_float2(sx1, sz1);
     vp_m1[k] = make_float2(vx1 - sx1, vz1 - sz1);
+}
+__global__ __launch_bounds__(256) void k_funstep(FGrid g, FWave w, int rb, float2 *__restrict__ vp_nt,
+                                                 float2 *__restrict__ vp_m1) {
+    funstep_body(g, w, rb, vp_nt, vp_m1, blockIdx3());
 }
 
 } // namespace ouc

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_fgrad(FGrid g, FWave src, int sbuf, FRe
     if (tile >= grad_tiles(g.nx, g.nz)) return;
     const int i0 = g.L + (tile / ntz) * kGradTX, j0 = g.L + (tile % ntz) * kGradTZ;
     const long so = (long)blockIdx.z * g.plane;
-    const KStrips rs = shot_strips(rc);
+    const KStrips rs = shot_strips(rc, blockIdx.z);
     const int tid = threadIdx.y * 64 + threadIdx.x;
     for (int e = tid; e < HX * HZ; e += 256) {
         const int xi = e / HZ, zj = e - xi * HZ;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_fgrad(FGrid g, FWave src, int sbuf, FRe
         float vx = 0.f, vz = 0.f, ux = 0.f, uz = 0.f;
         if (i < g.Xn && j < g.Zn) {
             recon_vin(g, src, rc, rs, sbuf, so, i, j, vx, vz);
-            load_vin(g, rcv, a, so, i, j, ux, uz);
+            load_vin(g, rcv, a, blockIdx.z, so, i, j, ux, uz);
         }
         sx_[xi][zj] = vx;
         sz_[xi][zj] = vz;

@@ -491,11 +491,11 @@ __device__ __forceinline__ float strip_get(const KStrips &s, int slot, int f, lo
 // dec[slot & 1]: the keyframe, or slot+1's decoded value minus slot+1's
 // residual (slot+1 must be the slot decoded before).  blockIdx.z is the shot,
 // with the given element strides between shots.
-static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slot, KStripStride d) {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
+__device__ __forceinline__ void strip_decode_body(const KStrips &s0, int slot, const KStripStride &d, const uint3 bid) {
+    const long q = (long)bid.x * blockDim.x + threadIdx.x;
+    const int f = bid.y;
     if (q >= s0.ncells) return;
-    const KStrips s = strips_of_shot(s0, d, blockIdx.z);
+    const KStrips s = strips_of_shot(s0, d, bid.z);
     const __half *f16 = s.f16;
     const int8_t *e8 = s.e8;
     const float *key = s.key;
@@ -515,6 +515,9 @@ static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slo
         }
     }
     dec[((long)(slot & 1) * RS_NF + f) * s.ncells + q] = v;
+}
+static __global__ __launch_bounds__(256) void k_strip_decode(KStrips s0, int slot, KStripStride d) {
+    strip_decode_body(s0, slot, d, make_uint3(blockIdx.x, blockIdx.y, blockIdx.z));
 }
 
 // The strip encoder alone: `steps` consecutive slots of field 0, x as

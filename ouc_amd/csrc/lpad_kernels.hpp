@@ -32,33 +32,37 @@ struct LbCell {
 // 2nd-order stress on pad cells (those with edge_dist < lpad).  V is read as
 // the interior's stress pass sees it (load_vin): on the receiver row of the
 // receiver wavefield that is the injected record.
-__global__ __launch_bounds__(256) void k_lpad_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
-                                                     FStep a) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void lpad_stress_body(const FGrid &g, const FWave &w, const KModel &m, const int2 *__restrict__ tiles,
+                                                     const FStep &a, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     if (i >= g.Xn || j >= g.Zn || !in_lpad(g, i, j)) return;
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const bool t_live = i >= 1 && i < g.Xn - 1 && j >= 1 && j < g.Zn - 1;
     const bool w_live = i < g.Xn - 1 && j < g.Zn - 1;
     float vx0, vz0, vxa, vza, vxb, vzb;
-    load_vin(g, w, a, so, i, j, vx0, vz0);
+    load_vin(g, w, a, bid.z, so, i, j, vx0, vz0);
     float gxx = 0.f, gzz = 0.f, gxz = 0.f, gzx = 0.f;
     if (t_live) {
-        load_vin(g, w, a, so, i - 1, j, vxa, vza);
-        load_vin(g, w, a, so, i, j - 1, vxb, vzb);
+        load_vin(g, w, a, bid.z, so, i - 1, j, vxa, vza);
+        load_vin(g, w, a, bid.z, so, i, j - 1, vxb, vzb);
         gxx = (vx0 - vxa) * g.idx;
         gzz = (vz0 - vzb) * g.idz;
     }
     if (w_live) {
-        load_vin(g, w, a, so, i, j + 1, vxa, vza);
-        load_vin(g, w, a, so, i + 1, j, vxb, vzb);
+        load_vin(g, w, a, bid.z, so, i, j + 1, vxa, vza);
+        load_vin(g, w, a, bid.z, so, i + 1, j, vxb, vzb);
         gxz = (vxa - vx0) * g.idz;
         gzx = (vzb - vz0) * g.idx;
     }
     w.th[k] = m.p[k0] * (gxx + gzz);
     w.om[k] = m.q[k0] * (gxz - gzx);
     w.gxx[k] = gxx; w.gzz[k] = gzz; w.gxz[k] = gxz; w.gzx[k] = gzx;
+}
+__global__ __launch_bounds__(256) void k_lpad_stress(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                     FStep a) {
+    lpad_stress_body(g, w, m, tiles, a, blockIdx3());
 }
 
 // Increment damping of the pad, 1 - beta ((lpad - d) / lpad)^2 at depth d: a
@@ -73,12 +77,12 @@ __device__ __forceinline__ float lb_gamma(const FGrid &g, int d) {
 // 2nd-order velocity update on the live pad cells (the network ring and the
 // ghost line are left to k_lb_ring / stay zero).  The P/S split is not carried
 // in the pad: nothing reads S there.
-__global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
-                                                       int rb) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void lpad_velocity_body(const FGrid &g, const FWave &w, const KModel &m, const int2 *__restrict__ tiles,
+                                                       int rb, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     if (i >= g.Xn || j >= g.Zn || !in_lpad(g, i, j)) return;
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k0 = (long)i * g.P + j, k = so + k0;
     const long P = g.P;
     const float *q = m.q;
@@ -103,6 +107,10 @@ __global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel 
         w.DV[1][k] = d;
         w.V[wb][1][k] = w.V[rb][1][k] + d;
     }
+}
+__global__ __launch_bounds__(256) void k_lpad_velocity(FGrid g, FWave w, KModel m, const int2 *__restrict__ tiles,
+                                                       int rb) {
+    lpad_velocity_body(g, w, m, tiles, rb, blockIdx3());
 }
 
 // Input channel `ch` of the network (0: D_x, 1: D_z, 2: txx, 3: tzz, 4: txz,
@@ -153,13 +161,13 @@ __device__ __forceinline__ int lb_map(const FGrid &g, int side, int cc, int a, i
 // ring cell advances them (a window holds other ring cells), so the new
 // increments go to dnew[shot][ncells] and k_lb_apply stores them.  With
 // wts == NULL the ring is held at zero here (the pad without the network).
-__global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, const LbCell *__restrict__ cells,
+__device__ __forceinline__ void lb_ring_body(const FGrid &g, const FWave &w, const KModel &m, const LbCell *__restrict__ cells,
                                                  int ncells, const float *__restrict__ wts, int rb, float sc,
-                                                 float *__restrict__ dnew) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 float *__restrict__ dnew, const uint3 bid) {
+    const int e = bid.x * blockDim.x + threadIdx.x;
     if (e >= ncells) return;
     const LbCell cl = cells[e];
-    const long so = (long)blockIdx.z * g.plane;
+    const long so = (long)bid.z * g.plane;
     const long k = so + (long)cl.i * g.P + cl.j;
     const int wb = 1 - rb;
     if (!wts) {
@@ -198,33 +206,42 @@ __global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, con
     float sg_out = 1.f;
     if (side == 1) sg_out = kLbXS[f];
     if (side == 3) sg_out = kLbZS[f];
-    dnew[(long)blockIdx.z * ncells + e] = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
+    dnew[(long)bid.z * ncells + e] = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
+}
+__global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, const LbCell *__restrict__ cells,
+                                                 int ncells, const float *__restrict__ wts, int rb, float sc,
+                                                 float *__restrict__ dnew) {
+    lb_ring_body(g, w, m, cells, ncells, wts, rb, sc, dnew, blockIdx3());
 }
 
 // The ring's new increments from k_lb_ring: D = dnew, V_new = V + D.
-__global__ __launch_bounds__(256) void k_lb_apply(FGrid g, FWave w, const LbCell *__restrict__ cells, int ncells,
-                                                  const float *__restrict__ dnew, int rb) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void lb_apply_body(const FGrid &g, const FWave &w, const LbCell *__restrict__ cells, int ncells,
+                                                  const float *__restrict__ dnew, int rb, const uint3 bid) {
+    const int e = bid.x * blockDim.x + threadIdx.x;
     if (e >= ncells) return;
     const LbCell cl = cells[e];
-    const long k = (long)blockIdx.z * g.plane + (long)cl.i * g.P + cl.j;
-    const float d = dnew[(long)blockIdx.z * ncells + e];
+    const long k = (long)bid.z * g.plane + (long)cl.i * g.P + cl.j;
+    const float d = dnew[(long)bid.z * ncells + e];
     w.DV[cl.f][k] = d;
     w.V[1 - rb][cl.f][k] = w.V[rb][cl.f][k] + d;
+}
+__global__ __launch_bounds__(256) void k_lb_apply(FGrid g, FWave w, const LbCell *__restrict__ cells, int ncells,
+                                                  const float *__restrict__ dnew, int rb) {
+    lb_apply_body(g, w, cells, ncells, dnew, rb, blockIdx3());
 }
 
 // Largest |V| of each shot over the pad (mon[2 shot]) and over the band of the
 // same width just inside it (mon[2 shot + 1]), as running maxima.  Float bits
 // compare as integers: |x| has a clear sign bit and NaN sorts above +inf.
-__global__ __launch_bounds__(256) void k_lb_monitor(FGrid g, FWave w, const int2 *__restrict__ tiles, int rb,
-                                                    unsigned int *__restrict__ mon) {
-    const int2 t0 = tiles[blockIdx.x];
+__device__ __forceinline__ void lb_monitor_body(const FGrid &g, const FWave &w, const int2 *__restrict__ tiles, int rb,
+                                                    unsigned int *__restrict__ mon, const uint3 bid) {
+    const int2 t0 = tiles[bid.x];
     const int j = t0.y + threadIdx.x, i = t0.x + threadIdx.y;
     unsigned int up = 0u, uf = 0u;
     if (i < g.Xn && j < g.Zn) {
         const int d = edge_dist(g, i, j);
         if (d < 2 * g.lpad) {
-            const long k = (long)blockIdx.z * g.plane + (long)i * g.P + j;
+            const long k = (long)bid.z * g.plane + (long)i * g.P + j;
             const unsigned int u = max(__float_as_uint(fabsf(w.V[rb][0][k])), __float_as_uint(fabsf(w.V[rb][1][k])));
             if (d < g.lpad) up = u;
             else uf = u;
@@ -235,9 +252,13 @@ __global__ __launch_bounds__(256) void k_lb_monitor(FGrid g, FWave w, const int2
         uf = max(uf, (unsigned int)__shfl_xor((int)uf, o));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(mon + 2 * blockIdx.z, up);
-        atomicMax(mon + 2 * blockIdx.z + 1, uf);
+        atomicMax(mon + 2 * bid.z, up);
+        atomicMax(mon + 2 * bid.z + 1, uf);
     }
+}
+__global__ __launch_bounds__(256) void k_lb_monitor(FGrid g, FWave w, const int2 *__restrict__ tiles, int rb,
+                                                    unsigned int *__restrict__ mon) {
+    lb_monitor_body(g, w, tiles, rb, mon, blockIdx3());
 }
 
 } // namespace ouc

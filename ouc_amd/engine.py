@@ -41,6 +41,8 @@ def lib():
     L.ouc_rtm_shot.restype = C.c_int
     L.ouc_rtm_batch.argtypes = [C.c_void_p, P(abi.Shot), C.c_int32, fp, fp, fp]
     L.ouc_rtm_batch.restype = C.c_int
+    L.ouc_rtm_grouped.argtypes = [P(C.c_void_p), C.c_int32, P(P(abi.Shot)), P(C.c_int32), P(fp), P(fp)]
+    L.ouc_rtm_grouped.restype = C.c_int
     L.ouc_get_timings.argtypes = [C.c_void_p, P(abi.Timings)]
     L.ouc_get_timings.restype = C.c_int
     L.ouc_set_profiling.argtypes = [C.c_void_p, C.c_int32]
@@ -286,6 +288,36 @@ class Engine:
         t = abi.Timings()
         self._check(lib().ouc_get_timings(self._h, C.byref(t)))
         return {name: getattr(t, name) for name, _ in abi.Timings._fields_}
+
+
+def rtm_grouped(engines, shot_lists):
+    """Batched RTM over several grids in shared launches (ouc_rtm_grouped):
+    engines[i] (its own grid and model, fused engine) runs shot_lists[i].  Every
+    kernel of a time step is one launch over the shots of all grids.  Returns
+    one dict(stack_pp, stack_ps) per engine, equal to that engine's rtm_batch of
+    its shots.  OUC_NUMERIC_FALLBACK (a learned boundary diverged; the shots ran
+    again on CPML) sets fell_back on the first engine and warns."""
+    n = len(engines)
+    if n != len(shot_lists):
+        raise ValueError("one shot list per engine")
+    hs = (C.c_void_p * n)(*[e._h for e in engines])
+    arrs, keep = [], []
+    for e, shots in zip(engines, shot_lists):
+        a = (abi.Shot * max(1, len(shots)))()
+        for k, sh in enumerate(shots):
+            s, w = _shot_struct(sh, e.p.nt)
+            a[k] = s
+            keep.append(w)
+        arrs.append(a)
+    sp = (C.POINTER(abi.Shot) * n)(*[C.cast(a, C.POINTER(abi.Shot)) for a in arrs])
+    ns = (C.c_int32 * n)(*[len(s) for s in shot_lists])
+    outs = [{"stack_pp": np.zeros((e.p.nx, e.p.nz), np.float32), "stack_ps": np.zeros((e.p.nx, e.p.nz), np.float32)}
+            for e in engines]
+    fp = C.POINTER(C.c_float)
+    pp = (fp * n)(*[abi.fptr(o["stack_pp"]) for o in outs])
+    ps = (fp * n)(*[abi.fptr(o["stack_ps"]) for o in outs])
+    engines[0]._check(lib().ouc_rtm_grouped(hs, n, sp, ns, pp, ps))
+    return outs
 
 
 def write_image_files(directory, pp_lap, ps_lap, shot):

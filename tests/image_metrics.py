@@ -98,5 +98,5 @@ def bar_param(key, misses, bar, *values, id=None):
 
     marks = []
     if key in misses:
-        marks = [pytest.mark.xfail(strict=True, reason=f"measured {misses[key]:.1e} > bar {bar:.0e}")]
+        marks = [pytest.mark.xfail(strict=True, reason=f"measured {misses[key]:.3e} > bar {bar:.0e}")]
     return pytest.param(*values, marks=marks, id=id)

@@ -104,3 +104,4 @@ def test_argument_checks_need_no_gpu():
     v = np.zeros(4, np.float32)
     assert lib().ouc_model_update(0, 0, abi.fptr(v), abi.fptr(v), 30.0, 0, abi.fptr(v)) == abi.OUC_INVALID_ARG
     assert lib().ouc_fwi_gradient(None, None, 1, None, None, None, None, None) == abi.OUC_INVALID_ARG
+    assert lib().ouc_rtm_grouped(None, 0, None, None, None, None) == abi.OUC_INVALID_ARG

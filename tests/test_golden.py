@@ -29,30 +29,29 @@ def test_oracle_reproduces_golden_seismograms(oracle, gold):
 
 
 GOLD_MISSES = {
-    ('plain', 'vz'): 1.29e-05,
-    ('plain', 'vpz'): 1.65e-05,
-    ('plain', 'faithful_img_pp_whole'): 3.38e-04,
-    ('plain', 'faithful_img_ps_whole'): 3.32e-04,
-    ('plain', 'faithful_lap_pp_whole'): 1.85e-04,
-    ('plain', 'faithful_lap_pp_below8'): 1.05e-04,
-    ('plain', 'faithful_lap_ps_whole'): 1.82e-04,
-    ('plain', 'faithful_lap_ps_below8'): 1.08e-04,
-    ('plain', 'exact_img_pp_whole'): 6.35e-04,
-    ('plain', 'exact_img_ps_whole'): 5.16e-04,
-    ('plain', 'exact_lap_pp_whole'): 2.27e-03,
-    ('plain', 'exact_lap_pp_below8'): 3.72e-04,
-    ('plain', 'exact_lap_ps_whole'): 1.99e-03,
-    ('plain', 'exact_lap_ps_below8'): 3.26e-04,
-    ('fused', 'vpz'): 1.12e-04,
-    ('fused', 'vsx'): 5.90e-05,
-    ('fused', 'vsz'): 1.98e-04,
-    ('fused', 'faithful_img_ps_whole'): 1.60e-04,
-    ('fused', 'exact_img_pp_whole'): 1.43e-04,
-    ('fused', 'exact_img_ps_whole'): 2.51e-04,
-    ('fused', 'exact_lap_pp_whole'): 8.59e-04,
-    ('fused', 'exact_lap_pp_below8'): 1.58e-04,
-    ('fused', 'exact_lap_ps_whole'): 7.50e-04,
-    ('fused', 'exact_lap_ps_below8'): 1.37e-04,
+    ('plain', 'vz'): 1.240e-05,
+    ('plain', 'vpz'): 1.580e-05,
+    ('plain', 'faithful_img_pp_whole'): 2.710e-04,
+    ('plain', 'faithful_img_ps_whole'): 4.660e-04,
+    ('plain', 'faithful_lap_pp_whole'): 1.720e-04,
+    ('plain', 'faithful_lap_ps_whole'): 1.690e-04,
+    ('plain', 'exact_img_pp_whole'): 5.100e-04,
+    ('plain', 'exact_img_ps_whole'): 7.220e-04,
+    ('plain', 'exact_lap_pp_whole'): 2.040e-03,
+    ('plain', 'exact_lap_pp_below8'): 2.790e-04,
+    ('plain', 'exact_lap_ps_whole'): 1.780e-03,
+    ('plain', 'exact_lap_ps_below8'): 2.470e-04,
+    ('fused', 'vpz'): 3.870e-05,
+    ('fused', 'vsx'): 1.000e-04,
+    ('fused', 'vsz'): 6.830e-05,
+    ('fused', 'faithful_img_pp_whole'): 1.002e-04,
+    ('fused', 'faithful_img_ps_whole'): 1.430e-04,
+    ('fused', 'exact_img_pp_whole'): 1.880e-04,
+    ('fused', 'exact_img_ps_whole'): 2.230e-04,
+    ('fused', 'exact_lap_pp_whole'): 9.320e-04,
+    ('fused', 'exact_lap_pp_below8'): 1.760e-04,
+    ('fused', 'exact_lap_ps_whole'): 8.140e-04,
+    ('fused', 'exact_lap_ps_below8'): 1.540e-04,
 }
 _runs = {}
 QUANTITIES = (["vx", "vz", "vpx", "vpz", "vsx", "vsz"]

@@ -28,68 +28,68 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # Measured errors of the cases that miss the bar: (run, shot, kind, region) -> relative L2.
 MISSES = {
-    ('rtm1024-plain', 0, 'img_pp', 'whole'): 2.93e-03,
-    ('rtm1024-plain', 0, 'img_pp', 'top'): 7.41e-03,
-    ('rtm1024-plain', 0, 'img_ps', 'whole'): 8.48e-03,
-    ('rtm1024-plain', 0, 'img_ps', 'below8'): 5.10e-04,
-    ('rtm1024-plain', 0, 'img_ps', 'near'): 4.62e-04,
-    ('rtm1024-plain', 0, 'img_ps', 'top'): 7.16e-03,
-    ('rtm1024-plain', 0, 'lap_pp', 'whole'): 2.84e-03,
-    ('rtm1024-plain', 0, 'lap_pp', 'below8'): 2.78e-03,
-    ('rtm1024-plain', 0, 'lap_pp', 'near'): 2.78e-03,
-    ('rtm1024-plain', 0, 'lap_pp', 'top'): 2.18e-03,
-    ('rtm1024-plain', 0, 'lap_ps', 'whole'): 2.79e-03,
-    ('rtm1024-plain', 0, 'lap_ps', 'below8'): 2.02e-03,
-    ('rtm1024-plain', 0, 'lap_ps', 'near'): 2.02e-03,
-    ('rtm1024-plain', 0, 'lap_ps', 'top'): 2.17e-03,
-    ('rtm1024-fused', 0, 'img_pp', 'whole'): 1.26e-03,
-    ('rtm1024-fused', 0, 'img_pp', 'top'): 3.00e-03,
-    ('rtm1024-fused', 0, 'img_ps', 'whole'): 4.39e-03,
-    ('rtm1024-fused', 0, 'img_ps', 'below8'): 5.48e-04,
-    ('rtm1024-fused', 0, 'img_ps', 'near'): 5.26e-04,
-    ('rtm1024-fused', 0, 'img_ps', 'top'): 4.39e-03,
-    ('rtm1024-fused', 0, 'lap_pp', 'whole'): 1.50e-03,
-    ('rtm1024-fused', 0, 'lap_pp', 'below8'): 3.31e-03,
-    ('rtm1024-fused', 0, 'lap_pp', 'near'): 3.31e-03,
-    ('rtm1024-fused', 0, 'lap_pp', 'top'): 1.33e-03,
-    ('rtm1024-fused', 0, 'lap_ps', 'whole'): 1.47e-03,
-    ('rtm1024-fused', 0, 'lap_ps', 'below8'): 2.42e-03,
-    ('rtm1024-fused', 0, 'lap_ps', 'near'): 2.42e-03,
-    ('rtm1024-fused', 0, 'lap_ps', 'top'): 1.32e-03,
-    ('rtm2048-fused-fp16', 0, 'img_pp', 'whole'): 3.11e-03,
-    ('rtm2048-fused-fp16', 0, 'img_pp', 'below8'): 5.67e-03,
-    ('rtm2048-fused-fp16', 0, 'img_pp', 'top'): 3.10e-03,
-    ('rtm2048-fused-fp16', 0, 'img_ps', 'whole'): 9.97e-03,
-    ('rtm2048-fused-fp16', 0, 'img_ps', 'below8'): 6.27e-03,
-    ('rtm2048-fused-fp16', 0, 'img_ps', 'near'): 7.69e-04,
-    ('rtm2048-fused-fp16', 0, 'img_ps', 'top'): 9.31e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_pp', 'whole'): 5.00e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_pp', 'below8'): 7.99e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_pp', 'near'): 9.08e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_pp', 'top'): 2.57e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_ps', 'whole'): 4.97e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_ps', 'below8'): 7.96e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_ps', 'near'): 5.86e-03,
-    ('rtm2048-fused-fp16', 0, 'lap_ps', 'top'): 2.57e-03,
-    ('rtm2048-fused-fp16', 32, 'img_pp', 'whole'): 5.86e-02,
-    ('rtm2048-fused-fp16', 32, 'img_pp', 'below8'): 3.39e-01,
-    ('rtm2048-fused-fp16', 32, 'img_pp', 'top'): 5.87e-02,
-    ('rtm2048-fused-fp16', 32, 'img_ps', 'whole'): 2.08e-02,
-    ('rtm2048-fused-fp16', 32, 'img_ps', 'below8'): 3.50e-01,
-    ('rtm2048-fused-fp16', 32, 'img_ps', 'near'): 8.12e-04,
-    ('rtm2048-fused-fp16', 32, 'img_ps', 'top'): 3.51e-02,
-    ('rtm2048-fused-fp16', 32, 'lap_pp', 'whole'): 8.86e-02,
-    ('rtm2048-fused-fp16', 32, 'lap_pp', 'below8'): 1.42e-01,
-    ('rtm2048-fused-fp16', 32, 'lap_pp', 'near'): 9.39e-03,
-    ('rtm2048-fused-fp16', 32, 'lap_pp', 'top'): 1.18e-02,
-    ('rtm2048-fused-fp16', 32, 'lap_ps', 'whole'): 9.00e-02,
-    ('rtm2048-fused-fp16', 32, 'lap_ps', 'below8'): 1.37e-01,
-    ('rtm2048-fused-fp16', 32, 'lap_ps', 'near'): 5.15e-03,
-    ('rtm2048-fused-fp16', 32, 'lap_ps', 'top'): 1.18e-02,
-    ('stack', 'pp', 'whole'): 7.51e-02,
-    ('stack', 'pp', 'near'): 9.89e-02,
-    ('stack', 'ps', 'whole'): 7.87e-02,
-    ('stack', 'ps', 'near'): 9.72e-02,
+    ('rtm1024-plain', 0, 'img_pp', 'whole'): 5.300e-03,
+    ('rtm1024-plain', 0, 'img_pp', 'top'): 9.340e-03,
+    ('rtm1024-plain', 0, 'img_ps', 'whole'): 1.240e-02,
+    ('rtm1024-plain', 0, 'img_ps', 'below8'): 5.250e-04,
+    ('rtm1024-plain', 0, 'img_ps', 'near'): 4.560e-04,
+    ('rtm1024-plain', 0, 'img_ps', 'top'): 1.010e-02,
+    ('rtm1024-plain', 0, 'lap_pp', 'whole'): 3.470e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'below8'): 3.050e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'near'): 3.050e-03,
+    ('rtm1024-plain', 0, 'lap_pp', 'top'): 2.150e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'whole'): 3.410e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'below8'): 2.230e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'near'): 2.230e-03,
+    ('rtm1024-plain', 0, 'lap_ps', 'top'): 2.130e-03,
+    ('rtm1024-fused', 0, 'img_pp', 'whole'): 2.140e-03,
+    ('rtm1024-fused', 0, 'img_pp', 'top'): 3.310e-03,
+    ('rtm1024-fused', 0, 'img_ps', 'whole'): 5.690e-03,
+    ('rtm1024-fused', 0, 'img_ps', 'below8'): 5.380e-04,
+    ('rtm1024-fused', 0, 'img_ps', 'near'): 5.150e-04,
+    ('rtm1024-fused', 0, 'img_ps', 'top'): 3.450e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'whole'): 1.950e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'below8'): 3.350e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'near'): 3.350e-03,
+    ('rtm1024-fused', 0, 'lap_pp', 'top'): 1.370e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'whole'): 1.920e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'below8'): 2.440e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'near'): 2.440e-03,
+    ('rtm1024-fused', 0, 'lap_ps', 'top'): 1.360e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'whole'): 2.380e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'below8'): 8.860e-03,
+    ('rtm2048-fused-fp16', 0, 'img_pp', 'top'): 2.380e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'whole'): 5.580e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'below8'): 9.820e-03,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'near'): 6.630e-04,
+    ('rtm2048-fused-fp16', 0, 'img_ps', 'top'): 8.630e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'whole'): 6.960e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'below8'): 1.240e-02,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'near'): 7.270e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_pp', 'top'): 2.450e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'whole'): 7.100e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'below8'): 1.270e-02,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'near'): 4.720e-03,
+    ('rtm2048-fused-fp16', 0, 'lap_ps', 'top'): 2.450e-03,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'whole'): 1.470e-01,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'below8'): 7.060e-01,
+    ('rtm2048-fused-fp16', 32, 'img_pp', 'top'): 1.480e-01,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'whole'): 3.350e-02,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'below8'): 7.260e-01,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'near'): 8.130e-04,
+    ('rtm2048-fused-fp16', 32, 'img_ps', 'top'): 4.120e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'whole'): 1.560e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'below8'): 2.490e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'near'): 9.220e-03,
+    ('rtm2048-fused-fp16', 32, 'lap_pp', 'top'): 2.680e-02,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'whole'): 1.570e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'below8'): 2.380e-01,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'near'): 5.100e-03,
+    ('rtm2048-fused-fp16', 32, 'lap_ps', 'top'): 2.680e-02,
+    ('stack', 'pp', 'whole'): 1.330e-01,
+    ('stack', 'pp', 'near'): 2.010e-01,
+    ('stack', 'ps', 'whole'): 1.370e-01,
+    ('stack', 'ps', 'near'): 1.960e-01,
 }
 
 RUNS = {

@@ -37,8 +37,9 @@ def test_fused_batch_is_sum_of_single_shots(oracle):
 
 
 FWD_MISSES = {
-    'vsx': 1.03e-04,
-    'vsz': 8.23e-05,
+    'vpz': 1.243e-05,
+    'vsx': 6.990e-05,
+    'vsz': 1.250e-04,
 }
 FWD_COMP = ["vx", "vz", "vpx", "vpz", "vsx", "vsz"]
 _fwd = {}

@@ -1,0 +1,125 @@
+"""Shots of several grid sizes in one launch (ouc_rtm_grouped).
+
+Every kernel of a time step is one launch over the shots of all grids of a
+group; each grid's stack must equal, bit for bit, what that grid's own batched
+run (ouc_rtm_batch) gives for its shots.  The grids, step counts and shot
+positions differ between contexts, and shots within a context have uneven
+step counts, so the per-context step offsets, strip slots and start steps are
+all exercised."""
+import numpy as np
+import pytest
+
+from ouc_amd import abi, velocity_models
+from ouc_amd.engine import Engine, OucError, rtm_grouped
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(96, 48, 300), (160, 80, 420), (128, 64, 360)]
+
+
+def _engines(strips, sem, learned=False):
+    out = []
+    for k, (nx, nz, nt) in enumerate(GRIDS):
+        vp, vs, _ = velocity_models.layered_random(nx, nz, seed=20 + k)
+        if learned:
+            p = abi.make_params(nx, nz, 30, 5, nt, 5.0, 5.0, 4e-4, R=1e-3, semantics=sem, strip_precision=strips,
+                                engine=abi.ENGINE_FUSED, boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
+        else:
+            p = abi.make_params(nx, nz, 20, 5, nt, 5.0, 5.0, 4e-4, R=1e-3, semantics=sem, strip_precision=strips,
+                                engine=abi.ENGINE_FUSED)
+        e = Engine(p, vp, vs)
+        if learned:
+            from ouc_amd.learned import load_weights
+
+            e.set_learned_boundary(load_weights())
+        out.append(e)
+    return out
+
+
+def _shots():
+    lists = []
+    for k, (nx, nz, nt) in enumerate(GRIDS):
+        xs = [nx // 5, nx // 2, 4 * nx // 5][: 2 + (k % 2)]
+        sh = [dict(sx=x, sz=0, rec_z=0, fm=22.0, mute=1) for x in xs]
+        sh[-1]["nt"] = nt - 40 * (k + 1)  # a shorter shot starts late within its context
+        lists.append(sh)
+    return lists
+
+
+def _check_equal(engines, lists, tag):
+    grouped = rtm_grouped(engines, lists)
+    for k, (e, shots) in enumerate(zip(engines, lists)):
+        own = e.rtm_batch(shots)
+        for key in ("stack_pp", "stack_ps"):
+            a, b = grouped[k][key], own[key]
+            # cells without illumination hold non-finite values in both; compare the bits
+            assert np.nanmax(np.abs(b[np.isfinite(b)])) > 0, (tag, k, key)
+            if a.tobytes() != b.tobytes():
+                m = np.isfinite(a) & np.isfinite(b)
+                d = float(np.max(np.abs(a[m] - b[m]))) if m.any() else float("nan")
+                pytest.fail(f"{tag}: grid {GRIDS[k]} {key} differs from its own batch (max |diff| {d:.3e}, "
+                            f"non-finite cells {int((~np.isfinite(a)).sum())} vs {int((~np.isfinite(b)).sum())})")
+    return grouped
+
+
+@pytest.mark.parametrize("strips,sem", [(abi.STRIP_FP32, abi.SEM_FAITHFUL), (abi.STRIP_FP16, abi.SEM_EXACT)],
+                         ids=["fp32-faithful", "fp16-exact"])
+def test_grouped_cpml_equals_each_grid_alone(strips, sem):
+    print(f"[case] grouped CPML grids {GRIDS} strips={strips} semantics={sem}", flush=True)
+    es = _engines(strips, sem)
+    try:
+        lists = _shots()
+        first = _check_equal(es, lists, "cpml")
+        # a second call replays the captured group loops
+        again = rtm_grouped(es, lists)
+        for k in range(len(es)):
+            assert again[k]["stack_pp"].tobytes() == first[k]["stack_pp"].tobytes()
+            assert again[k]["stack_ps"].tobytes() == first[k]["stack_ps"].tobytes()
+    finally:
+        for e in es:
+            e.close()
+
+
+def test_grouped_learned_boundary_equals_each_grid_alone():
+    print(f"[case] grouped learned boundary grids {GRIDS} fp16 strips", flush=True)
+    es = _engines(abi.STRIP_FP16, abi.SEM_FAITHFUL, learned=True)
+    try:
+        _check_equal(es, _shots(), "learned")
+        assert not getattr(es[0], "fell_back", False)
+    finally:
+        for e in es:
+            e.close()
+
+
+def test_grouped_run_with_an_idle_context_and_two_rounds():
+    """A context with no shots takes no part; one whose shots exceed its batch
+    capacity runs in rounds, the other contexts joining only the first."""
+    print("[case] grouped: empty shot list, capacity 1 on one grid", flush=True)
+    es = _engines(abi.STRIP_FP32, abi.SEM_FAITHFUL)
+    try:
+        es[1].set_batch_capacity(1)
+        lists = _shots()
+        lists[2] = []
+        grouped = rtm_grouped(es, lists)
+        assert not np.any(grouped[2]["stack_pp"])
+        for k in (0, 1):
+            own = es[k].rtm_batch(lists[k])
+            assert grouped[k]["stack_pp"].tobytes() == own["stack_pp"].tobytes()
+            assert grouped[k]["stack_ps"].tobytes() == own["stack_ps"].tobytes()
+    finally:
+        for e in es:
+            e.close()
+
+
+def test_grouped_refuses_mixed_contexts():
+    print("[case] grouped: mixed strip precision refused", flush=True)
+    nx, nz, nt = GRIDS[0]
+    vp, vs, _ = velocity_models.layered_random(nx, nz, seed=3)
+    p32 = abi.make_params(nx, nz, 20, 5, nt, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED)
+    p16 = abi.make_params(nx, nz, 20, 5, nt, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED, strip_precision=abi.STRIP_FP16)
+    with Engine(p32, vp, vs) as a, Engine(p16, vp, vs) as b:
+        shots = [dict(sx=40, sz=0, rec_z=0, fm=22.0)]
+        with pytest.raises(OucError, match="INVALID_ARG"):
+            rtm_grouped([a, b], [shots, shots])
+        with pytest.raises(OucError, match="INVALID_ARG"):
+            rtm_grouped([a, a], [shots, shots])
