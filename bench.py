@@ -55,7 +55,7 @@ def parse():
                     help="uneven serving batch instead: 32 shots on grids 512x256..4096x2048, nt 2000-8000, "
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
     ap.add_argument("--serve-shots", type=int, default=32)
-    ap.add_argument("--serve-schedule", choices=["sjf", "concurrent"], default="sjf")
+    ap.add_argument("--serve-schedule", choices=["sjf", "concurrent", "grouped"], default="sjf")
     ap.add_argument("--fwi", action="store_true",
                     help="FWI gradient throughput (shots/hour) on the same workload instead of RTM")
     ap.add_argument("--split-pml", action="store_true",
@@ -365,7 +365,8 @@ def serve_bench(a):
         "dtype": "f32", "data": "synthetic (seeded layered models per grid, random shots)",
         "config": {"workload": f"{len(jobs)} RTM shots, grids 512x256/1024x512/2048x1024/4096x2048, nt 2000-8000, "
                                "learned 8-cell boundary with CPML fallback, FP16 strips, autotuned tiles, "
-                               f"per-grid batches, schedule {a.serve_schedule}"},
+                               + ("all grids in shared launches per time step" if a.serve_schedule == "grouped"
+                                  else f"per-grid batches, schedule {a.serve_schedule}")},
         "latency_ms_p50": s["latency_ms_p50"], "latency_ms_p95": s["latency_ms_p95"], "groups": s["groups"],
     }
     print(json.dumps(line))

@@ -6,9 +6,12 @@ group the shots run as batches of the fused engine: the context holds the
 group's largest step count, and a shorter shot starts late in the batch (its
 wavelet is zero before its start and it images only its own steps), so every
 shot's image equals the image of its own run.  Each group's batches replay
-their captured time-loop graphs with a tile from the autotuner's cache, and
-the groups run concurrently, one host thread and one HIP stream per context,
-so small grids fill the GPU beside large ones.
+their captured time-loop graphs with a tile from the autotuner's cache.  The
+groups run one after another ("sjf"), concurrently from one host thread and
+stream each ("concurrent"), or together in shared launches ("grouped",
+ouc_rtm_grouped): every kernel of a time step is one launch over the shots of
+all grids, so small grids fill the GPU beside large ones without competing
+streams.
 
 Latency of a shot = time from the common start to the end of the batch that
 holds it; throughput = shots / wall time of the whole job.
@@ -20,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import abi, tuning
-from .engine import Engine
+from .engine import Engine, rtm_grouped
 from .velocity_models import layered_random
 
 GRIDS = ((512, 256), (1024, 512), (2048, 1024), (4096, 2048))
@@ -75,7 +78,11 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, sc
 
     schedule "sjf": groups run one after another, least work (cells x steps x
     shots) first -- small grids finish early instead of queueing behind large
-    kernels; "concurrent": all groups at once, one host thread and stream each.
+    kernels; "concurrent": all groups at once, one host thread and stream each;
+    "grouped": all groups in shared launches (one launch per kernel and time
+    step over every grid's shots), a shot's latency being the end of the call;
+    the grouped launch refuses grids beyond 420 steps or 160 x 80 cells (see
+    ouc_rtm_grouped).
     The groups' device buffers share `mem_budget_gb`: when all groups' shots do
     not fit at once, every group's batch shrinks by the same factor (the rest
     of its shots run as later batches)."""
@@ -129,9 +136,23 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, sc
         results[grid] = GroupResult(grid, len(js), js[0].nt, tile, wall, e.timings()["fallback_shots"],
                                     out["stack_pp"])
 
+    def run_grouped(t_start):
+        order = sorted(engines)
+        es = [engines[g][0] for g in order]
+        outs = rtm_grouped(es, [[j.shot for j in engines[g][1]] for g in order])
+        wall = time.perf_counter() - t_start
+        for g, e, out in zip(order, es, outs):
+            js = engines[g][1]
+            for j in js:
+                lat[j.id] = wall * 1e3
+            results[g] = GroupResult(g, len(js), js[0].nt, engines[g][2], wall, e.timings()["fallback_shots"],
+                                     out["stack_pp"])
+
     def run_all():
         t_start = time.perf_counter()
-        if concurrent:
+        if schedule == "grouped":
+            run_grouped(t_start)
+        elif concurrent:
             th = [threading.Thread(target=run, args=(g, t_start)) for g in engines]
             for t in th:
                 t.start()

@@ -123,3 +123,18 @@ def test_grouped_refuses_mixed_contexts():
             rtm_grouped([a, b], [shots, shots])
         with pytest.raises(OucError, match="INVALID_ARG"):
             rtm_grouped([a, a], [shots, shots])
+
+
+def test_grouped_refuses_groups_beyond_the_validated_envelope():
+    """Groups past 420 steps or 160 x 80 interior cells are refused before any
+    work (a larger group ended its process with a host fault whose cause is
+    not found)."""
+    print("[case] grouped: 200x100 grid and a 421-step grid refused", flush=True)
+    vp, vs, _ = velocity_models.layered_random(200, 100, seed=4)
+    big = abi.make_params(200, 100, 20, 5, 300, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED)
+    vp2, vs2, _ = velocity_models.layered_random(96, 48, seed=4)
+    long_ = abi.make_params(96, 48, 20, 5, 421, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED)
+    with Engine(big, vp, vs) as a, Engine(long_, vp2, vs2) as b:
+        for e in (a, b):
+            with pytest.raises(OucError, match="UNSUPPORTED"):
+                rtm_grouped([e], [[dict(sx=40, sz=0, rec_z=0, fm=22.0)]])

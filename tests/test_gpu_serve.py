@@ -54,8 +54,10 @@ def test_serving_groups_equal_standalone_runs(tmp_path, monkeypatch):
     jobs = serve.make_jobs(n=6, seed=3, grids=((96, 48), (128, 64)), nt_range=(200, 320))
     lat, wall, groups = serve.serve(jobs, learned=True, strips=abi.STRIP_FP32, warm=True, schedule="concurrent")
     lat2, wall2, groups2 = serve.serve(jobs, learned=True, strips=abi.STRIP_FP32, warm=False, schedule="sjf")
-    for a, b in zip(groups, groups2):
+    lat3, wall3, groups3 = serve.serve(jobs, learned=True, strips=abi.STRIP_FP32, warm=True, schedule="grouped")
+    for a, b, c in zip(groups, groups2, groups3):
         np.testing.assert_array_equal(a.stack_pp, b.stack_pp)
+        np.testing.assert_array_equal(a.stack_pp, c.stack_pp)  # all grids in shared launches
     s = serve.summary(jobs, lat, wall, groups)
     print(f"[serve] {s['shots']} shots in {s['wall_s']:.2f} s, p50 {s['latency_ms_p50']:.1f} ms, "
           f"p95 {s['latency_ms_p95']:.1f} ms, groups {s['groups']}", flush=True)
