@@ -238,9 +238,10 @@ ouc_status ouc_set_tile(ouc_ctx *c, const ouc_tile *t);
 ouc_status ouc_get_tile(ouc_ctx *c, ouc_tile *t);
 
 /* Autotuner: times the interior kernel on this context's grid with nshots
- * shots, `reps` launches per candidate (core_cols {8, 16, 32, 64, 128, 256}
+ * shots, `reps` launches per candidate (core_cols {8, 16, 32, 64, 128, W}
  * x prefetch {1, 2, 3}: 18 candidates, ms_per_launch[18] optional), selects
- * the fastest and returns it in *best. */
+ * the fastest and returns it in *best.  W is the width at which one round of
+ * the blocks the device holds at once covers the launch. */
 ouc_status ouc_autotune(ouc_ctx *c, int32_t nshots, int32_t reps, ouc_tile *best, float *ms_per_launch);
 
 /* Streaming-rate probe (no counterpart in the reference; a measurement aid):

@@ -1727,16 +1727,46 @@ void fused_get_tile(ouc_ctx *c, int *cols, int *pf) {
     *pf = fs(c)->pf;
 }
 
+// Blocks of the forward core with prefetch depth pf one compute unit holds at once.
+template <int N>
+int core_blocks_per_cu(int pf) {
+    int nb = 0;
+    if constexpr (N == 5 || N == 6) {
+        if (pf == 1) OUC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_core<N, CORE_STEP, 1>, 256, 0));
+        else if (pf == 3) OUC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_core<N, CORE_STEP, 3>, 256, 0));
+        else OUC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_core<N, CORE_STEP, kCorePF>, 256, 0));
+    } else {
+        OUC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_core<N, CORE_STEP, kCorePF>, 256, 0));
+    }
+    return nb;
+}
+
+// Core width at which one round of resident blocks covers the launch: the
+// (z block, shot) strips times the tiles per strip fit in the blocks the
+// device holds at once (several rounds only when the strips alone exceed it).
+// A launch of slightly more blocks than one round waits a whole extra block
+// time for the excess, so this width avoids the partly filled last round.
+int core_fit_cols(const ouc_ctx *c, const FGrid &f, int nshots) {
+    int cus = 0, per_cu = 0;
+    OUC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    OUC_FDISPATCH(f.N, per_cu = core_blocks_per_cu<NN>(kCorePF));
+    const long strips = (long)core_zblocks(f.cz0, f.cz1) * nshots;
+    const long tiles = std::max(1L, (long)cus * std::max(1, per_cu) / std::max(1L, strips));
+    const long w = f.cx1 - f.cx0;
+    return (int)std::min(1024L, std::max(8L, (w + tiles - 1) / tiles));
+}
+
 // Times k_core (forward mode) over `reps` launches for each candidate tile on
 // this grid with `nshots` shots; keeps the fastest.  ms[k] gets candidate k's
-// mean launch time (candidates: cols {8, 16, 32, 64, 128, 256} x pf {1, 2, 3}).
+// mean launch time (candidates: cols {8, 16, 32, 64, 128, one-round width
+// (core_fit_cols)} x pf {1, 2, 3}).
 void fused_autotune(ouc_ctx *c, int nshots, int reps, int *best_cols, int *best_pf, float *ms) {
     if (nshots < 1 || reps < 1) throw Error(OUC_INVALID_ARG, "nshots and reps must be positive");
     ensure(c, nshots);
     FusedState *st = fs(c);
     hipStream_t s = c->stream;
     OUC_HIP(hipMemsetAsync(st->mem_s.ptr, 0, st->mem_s.bytes, s));
-    const int cols[6] = {8, 16, 32, 64, 128, 256}, pfs[3] = {1, 2, 3};
+    const int cols[6] = {8, 16, 32, 64, 128, core_fit_cols(c, st->g, nshots)}, pfs[3] = {1, 2, 3};
     hipEvent_t e0, e1;
     OUC_HIP(hipEventCreate(&e0));
     OUC_HIP(hipEventCreate(&e1));

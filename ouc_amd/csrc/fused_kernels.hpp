@@ -227,6 +227,16 @@ __device__ __forceinline__ float2 ldb2(const float2 *base, unsigned boff) {
 __device__ __forceinline__ void stb2(float2 *base, unsigned boff, float2 v) {
     *reinterpret_cast<float2 *>(reinterpret_cast<char *>(base) + boff) = v;
 }
+// Streaming store (non-temporal hint) for state the launch does not read
+// back: the core's new V, increments and S part.  Measured about 3% faster
+// than a plain store on the forward core at 2048 x 1024, 8 shots.
+typedef float ntf2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stb2s(float2 *base, unsigned boff, float2 v) {
+    ntf2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<ntf2 *>(reinterpret_cast<char *>(base) + boff));
+}
 
 // MODE 0 steps a wavefield (the forward source run, or the receiver run
 // without imaging); MODE 2 is the receiver run with the imaging condition
@@ -442,12 +452,12 @@ __device__ __forceinline__ void core_body(const FGrid &g, const FWave &w, const 
             const float sg = RECON ? -1.f : 1.f; // the reverse step runs the increments backwards
             const float dx = st6[0] + sg * dt2 * (tx + hx + 2.f * Kx); vx = vx + sg * dx;
             const float dz = st6[1] + sg * dt2 * (zq[0][0] + hz + 2.f * Kz); vz = vz + sg * dz;
-            stb2(DVp, k, make_float2(dx, dz));
-            stb2(Vw, k, make_float2(vx, vz));
+            stb2s(DVp, k, make_float2(dx, dz));
+            stb2s(Vw, k, make_float2(vx, vz));
             const float ex = st6[4] + sg * dt2 * (hx + Kx); const float sx = st6[2] + sg * ex;
             const float ez = st6[5] + sg * dt2 * (hz + Kz); const float sz = st6[3] + sg * ez;
-            stb2(DSp, k, make_float2(ex, ez));
-            stb2(Sp, k, make_float2(sx, sz));
+            stb2s(DSp, k, make_float2(ex, ez));
+            stb2s(Sp, k, make_float2(sx, sz));
             if (RECON) {
                 if (rc.vp) // the source P level for the imaging condition (none for the FWI gradient)
                     stb2(rc.vp + so, k, make_float2(vx - sx, vz - sz));

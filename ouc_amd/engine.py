@@ -233,11 +233,13 @@ class Engine:
 
     def autotune(self, nshots, reps=5):
         """Times the 18 candidate tiles on this grid, keeps the fastest;
-        returns ((core_cols, prefetch), {(cols, pf): ms per launch})."""
+        returns ((core_cols, prefetch), {(cols, pf): ms per launch}).  The sixth
+        width, labelled "fit", is the one at which a single round of resident
+        blocks covers the launch (ouc_autotune)."""
         t = abi.Tile()
         ms = np.zeros(18, np.float32)
         self._check(lib().ouc_autotune(self._h, int(nshots), int(reps), C.byref(t), abi.fptr(ms)))
-        cands = [(c, p) for c in (8, 16, 32, 64, 128, 256) for p in (1, 2, 3)]
+        cands = [(c, p) for c in (8, 16, 32, 64, 128, "fit") for p in (1, 2, 3)]
         return (int(t.core_cols), int(t.prefetch)), {k: float(v) for k, v in zip(cands, ms)}
 
     def set_graphs(self, on=True):

@@ -40,7 +40,7 @@ def test_tiles_give_identical_results_and_tuner_picks_a_candidate(tmp_path, monk
         monkeypatch.setenv("OUC_TILE_CACHE", str(tmp_path / "tiles.json"))
         cols, pf, cached = tuning.tuned(e, len(shots), reps=3)
         assert not cached and (cols, pf) == e.tile()
-        assert cols in (8, 16, 32, 64, 128, 256) and pf in (1, 2, 3)
+        assert 8 <= cols <= 1024 and pf in (1, 2, 3)
         cols2, pf2, cached2 = tuning.tuned(e, len(shots))
         assert cached2 and (cols2, pf2) == (cols, pf)
         table = tuning.load_cache()[tuning.shape_key(tuning.device_arch(), p, len(shots))]["ms_per_launch"]
