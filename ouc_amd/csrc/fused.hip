@@ -43,14 +43,16 @@ namespace {
 template <int N, int MODE>
 void launch_core_mode(int pf, dim3 grid, hipStream_t s, const FGrid &f, const FWave &w, const float2 *pq,
                       const FStep &a, const FRecon &rc) {
+    const uint3 dims = make_uint3(grid.x, grid.y, grid.z); // x-tiles, z blocks, shots (XCD-ordered in the kernel)
+    const dim3 lin(xcd_blocks(dims));
     if constexpr (N == 5 || N == 6) {
         switch (pf) {
-        case 1: k_core<N, MODE, 1><<<grid, 256, 0, s>>>(f, w, pq, a, rc); return;
-        case 3: k_core<N, MODE, 3><<<grid, 256, 0, s>>>(f, w, pq, a, rc); return;
+        case 1: k_core<N, MODE, 1><<<lin, 256, 0, s>>>(f, w, pq, a, rc, dims); return;
+        case 3: k_core<N, MODE, 3><<<lin, 256, 0, s>>>(f, w, pq, a, rc, dims); return;
         default: break;
         }
     }
-    k_core<N, MODE, kCorePF><<<grid, 256, 0, s>>>(f, w, pq, a, rc);
+    k_core<N, MODE, kCorePF><<<lin, 256, 0, s>>>(f, w, pq, a, rc, dims);
 }
 
 // RECON = 1: the reconstruction; otherwise a wavefield step, with the imaging

@@ -485,10 +485,33 @@ __device__ __forceinline__ void core_body(const FGrid &g, const FWave &w, const 
         for (int cc = cbeg; cc < cend; ++cc) body(std::integral_constant<int, 0>{}, cc);
     }
 }
+// XCD-aware block order for a launch of dims.x x-tiles, dims.y z blocks and
+// dims.z shots.  Workgroups go round-robin to the 8 XCDs (workgroup b to XCD
+// b mod 8), each with its own L2.  The launch is one-dimensional with
+// 8 * ceil(T / 8) blocks (T = tiles x z blocks x shots); XCD k takes the k-th
+// eighth of the tiles in (x-tile, z block, shot) order, so the shots and z
+// blocks of an x-tile, which share the model planes and their halo rows, and
+// its neighbouring x-tiles meet in one L2.  Blocks past T return at once.
+constexpr int kXcds = 8;
+__host__ __device__ inline unsigned xcd_blocks(const uint3 dims) {
+    const unsigned t = dims.x * dims.y * dims.z;
+    return kXcds * ((t + kXcds - 1) / kXcds);
+}
+__device__ __forceinline__ bool xcd_bid(const uint3 dims, uint3 &bid) {
+    const unsigned t = dims.x * dims.y * dims.z, per = (t + kXcds - 1) / kXcds;
+    const unsigned b = blockIdx.x, q = (b % kXcds) * per + b / kXcds;
+    if (q >= t) return false;
+    const unsigned yz = dims.y * dims.z, r = q % yz;
+    bid = make_uint3(q / yz, r / dims.z, r % dims.z);
+    return true;
+}
+
 template <int N, int MODE, int PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_core(FGrid g, FWave w, const float2 *__restrict__ pq,
-                                              FStep a, FRecon rc) {
-    core_body<N, MODE, PF>(g, w, pq, a, rc, blockIdx3());
+                                              FStep a, FRecon rc, uint3 dims) {
+    uint3 bid;
+    if (!xcd_bid(dims, bid)) return;
+    core_body<N, MODE, PF>(g, w, pq, a, rc, bid);
 }
 
 // ---------------------------------------------------------------------------
