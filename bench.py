@@ -56,6 +56,9 @@ def parse():
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
     ap.add_argument("--serve-shots", type=int, default=32)
     ap.add_argument("--serve-schedule", choices=["sjf", "concurrent", "grouped"], default="sjf")
+    ap.add_argument("--serve-grids", choices=["full", "small"], default="full",
+                    help="full: 512x256..4096x2048, nt 2000-8000; small: 96x48..160x80, nt 200-420 "
+                         "(the grouped launch's validated envelope)")
     ap.add_argument("--fwi", action="store_true",
                     help="FWI gradient throughput (shots/hour) on the same workload instead of RTM")
     ap.add_argument("--split-pml", action="store_true",
@@ -354,7 +357,9 @@ def serve_bench(a):
     latency (a warm pass captures the graphs and fills the tile cache first)."""
     from ouc_amd import serve
 
-    jobs = serve.make_jobs(n=a.serve_shots)
+    small = a.serve_grids == "small"
+    grids, nt_range = (serve.SMALL_GRIDS, serve.SMALL_NT) if small else (serve.GRIDS, (2000, 8000))
+    jobs = serve.make_jobs(n=a.serve_shots, grids=grids, nt_range=nt_range)
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
     lat, wall, groups = serve.serve(jobs, schedule=a.serve_schedule, log=log)
     s = serve.summary(jobs, lat, wall, groups)
@@ -363,7 +368,8 @@ def serve_bench(a):
         "value": s["shots_per_hour"], "unit": "shots/hour", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded layered models per grid, random shots)",
-        "config": {"workload": f"{len(jobs)} RTM shots, grids 512x256/1024x512/2048x1024/4096x2048, nt 2000-8000, "
+        "config": {"workload": f"{len(jobs)} RTM shots, grids {'/'.join(f'{g[0]}x{g[1]}' for g in grids)}, "
+                               f"nt {nt_range[0]}-{nt_range[1]}, "
                                "learned 8-cell boundary with CPML fallback, FP16 strips, autotuned tiles, "
                                + ("all grids in shared launches per time step" if a.serve_schedule == "grouped"
                                   else f"per-grid batches, schedule {a.serve_schedule}")},

@@ -27,6 +27,11 @@ from .engine import Engine, rtm_grouped
 from .velocity_models import layered_random
 
 GRIDS = ((512, 256), (1024, 512), (2048, 1024), (4096, 2048))
+# Small grids and step counts inside the grouped launch's validated envelope
+# (at most 160 x 80 cells and 420 steps per grid): where launches are short
+# and one grid alone does not fill the GPU.
+SMALL_GRIDS = ((96, 48), (112, 56), (128, 64), (160, 80))
+SMALL_NT = (200, 420)
 
 
 @dataclass
