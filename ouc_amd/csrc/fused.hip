@@ -98,6 +98,8 @@ struct FusedState {
     // learned boundary
     DevBuf tiles_p, tiles_m, lbcells, lbw, mon;
     DevBuf lbd; // [cap][nlbcells]: the network ring's new increments of a step
+    DevBuf lbc; // [cap][lb_canon_floats(lbE)]: the network's input rows, staged per side
+    int lbE = 0; // positions per staged row: max(Xn, Zn)
     int ntiles_p = 0, ntiles_m = 0, nlbcells = 0;
     bool lb_have = false;  // composed filter present (else the ring is held at zero)
     int csx = kCoreSX, pf = kCorePF; // core tile
@@ -271,6 +273,14 @@ void build_learned(FusedState *st, const Grid &g, hipStream_t s) {
             if (!gx && !live_x) ring.push_back(LbCell{i, j, 0, 0});
             if (!gz && !live_z) ring.push_back(LbCell{i, j, 1, 0});
         }
+    // k_lb_ring reads its window from rows staged at canonical depths 0..kLbD-1,
+    // which covers every ring cell's window only at canonical depth 0
+    for (const LbCell &r : ring) {
+        const int SX = r.f == 0 ? 1 : 0, SZ = r.f == 1 ? 1 : 0;
+        const int a = std::min(std::min(r.i, g.Xn - 1 - SX - r.i), std::min(r.j, g.Zn - 1 - SZ - r.j));
+        if (a != 0) throw Error(OUC_UNSUPPORTED, "learned ring cell off the outermost ring");
+    }
+    st->lbE = std::max(g.Xn, g.Zn);
     st->nlbcells = (int)ring.size();
     st->lbcells.alloc(ring.size() * sizeof(LbCell));
     copy_h2d(s, st->lbcells.ptr, ring.data(), ring.size() * sizeof(LbCell));
@@ -291,6 +301,7 @@ size_t per_shot_bytes(const ouc_ctx *c, int fp16) {
              (size_t)((g.nt + kStripKey - 1) / kStripKey) * RS_NF * cells * sizeof(float) +
              (size_t)4 * RS_NF * cells * sizeof(float);
     b += (size_t)2 * g.nt * g.nx * sizeof(float) + (size_t)g.nt * sizeof(float) + g.nx * sizeof(int32_t);
+    if (g.learned) b += (size_t)lb_canon_floats(std::max(g.Xn, g.Zn)) * sizeof(float);
     return b;
 }
 
@@ -411,6 +422,7 @@ void ensure(ouc_ctx *c, int shots) {
     if (g.learned) {
         build_learned(st, g, c->stream);
         st->lbd.alloc((size_t)shots * st->nlbcells * sizeof(float));
+        st->lbc.alloc((size_t)shots * lb_canon_floats(st->lbE) * sizeof(float));
     }
 }
 
@@ -531,8 +543,10 @@ void wave_step_probed(ouc_ctx *c, FusedState *st, const Launch &L, FWave &w, con
         // the ring reads the pad's increments before the pad update advances them
         const float sc = f.dt2 * f.idx;
         const dim3 rg((st->nlbcells + 255) / 256, 1, B);
-        k_lb_ring<<<rg, 256, 0, s>>>(f, w, km, st->lbcells.as<LbCell>(), st->nlbcells,
-                                     st->lb_have ? st->lbw.f() : nullptr, a.rb, sc, st->lbd.f());
+        if (st->lb_have)
+            k_lb_canon<<<dim3((st->lbE + 255) / 256, 4 * kLbD, B), 256, 0, s>>>(f, w, km, sc, st->lbc.f(), st->lbE);
+        k_lb_ring<<<rg, 256, 0, s>>>(f, w, st->lbcells.as<LbCell>(), st->nlbcells, st->lb_have ? st->lbw.f() : nullptr,
+                                     a.rb, st->lbc.f(), st->lbE, st->lbd.f());
         if (st->lb_have) k_lb_apply<<<rg, 256, 0, s>>>(f, w, st->lbcells.as<LbCell>(), st->nlbcells, st->lbd.f(), a.rb);
         k_lpad_velocity<<<dim3(st->ntiles_p, 1, B), dim3(64, 4), 0, s>>>(f, w, km, st->tiles_p.as<int2>(), a.rb);
         if (a.t % kMonStride == 0)
@@ -1103,6 +1117,8 @@ GCtx make_gctx(ouc_ctx *c, const BatchIn &bi, int T) {
     x.lbrb = (st->nlbcells + 255) / 256;
     x.lbw = st->lb_have ? st->lbw.f() : nullptr;
     x.lbd = st->lbd.f();
+    x.lbc = st->lbc.f();
+    x.lbE = st->lbE;
     x.lbsc = st->g.dt2 * st->g.idx;
     x.mon_f = bi.mon_f;
     x.mon_r = bi.mon_r;
@@ -1200,6 +1216,8 @@ void run_group_batch(const std::vector<ouc_ctx *> &cs, const std::vector<const o
             if (nb) OUC_GDISPATCH(N, k_band_velocity_grp<NN, RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg))
         }
         if (learned) {
+            Lg = mk(tau, [&](int i, int) { return hc[i].lbw ? (hc[i].lbE + 255) / 256 * 4 * kLbD * hc[i].B : 0; }, nb);
+            if (nb) k_lb_canon_grp<RCV><<<nb, 256, 0, q>>>(tab, Lg);
             Lg = mk(tau, [&](int i, int) { return hc[i].lbrb * hc[i].B; }, nb);
             if (nb) k_lb_ring_grp<RCV><<<nb, 256, 0, q>>>(tab, Lg);
             Lg = mk(tau, [&](int i, int) { return hc[i].lbw ? hc[i].lbrb * hc[i].B : 0; }, nb);

@@ -39,6 +39,8 @@ struct GCtx {
     int nlbcells, lbrb; // network ring cells, their blocks per shot
     const float *lbw;  // composed filter, or NULL (ring held at zero)
     float *lbd;
+    float *lbc;        // the network's staged input rows, and their length
+    int lbE;
     float lbsc;
     unsigned int *mon_f, *mon_r;
     int nt, B, start, exact, edge, rblocks;
@@ -191,12 +193,18 @@ __global__ __launch_bounds__(256) void k_lpad_stress_grp(const GCtx *__restrict_
 }
 
 template <bool RCV>
+__global__ __launch_bounds__(256) void k_lb_canon_grp(const GCtx *__restrict__ tab, GLaunch L) {
+    int local;
+    const GCtx &c = gctx(tab, L, blockIdx.x, local);
+    lb_canon_body(c.g, RCV ? c.R : c.S, c.km, c.lbsc, c.lbc, c.lbE, gbid(local, (c.lbE + 255) / 256, 4 * kLbD));
+}
+
+template <bool RCV>
 __global__ __launch_bounds__(256) void k_lb_ring_grp(const GCtx *__restrict__ tab, GLaunch L) {
     int local;
     const GCtx &c = gctx(tab, L, blockIdx.x, local);
     const FStep a = g_wave(c, RCV, L.tau - c.start);
-    lb_ring_body(c.g, RCV ? c.R : c.S, c.km, c.lbcells, c.nlbcells, c.lbw, a.rb, c.lbsc, c.lbd,
-                 gbid(local, c.lbrb, 1));
+    lb_ring_body(c.g, RCV ? c.R : c.S, c.lbcells, c.nlbcells, c.lbw, a.rb, c.lbc, c.lbE, c.lbd, gbid(local, c.lbrb, 1));
 }
 
 template <bool RCV>

@@ -154,16 +154,59 @@ __device__ __forceinline__ int lb_map(const FGrid &g, int side, int cc, int a, i
     return oc;
 }
 
+// The network's input, staged per side in canonical coordinates: the five
+// canonical channels (channel map and mirror sign of lb_map applied) at depths
+// a = 0..kLbD-1 along the whole side, buf[shot][side][channel][a][position],
+// E = max(Xn, Zn) positions per row.  Every ring cell has canonical depth 0
+// (build_learned checks it), so its 7x7 window reads depths 0..3 only; each
+// staged value is read by up to 28 windows instead of being formed for each.
+constexpr int kLbD = 4;
+__host__ __device__ inline long lb_canon_floats(int E) { return 4L * kLbIn * kLbD * E; } // per shot
+__device__ __forceinline__ int lb_side_len(const FGrid &g, int side) { return side < 2 ? g.Zn : g.Xn; }
+
+// One staged position per thread: bid.x * blockDim.x + lane along the side,
+// bid.y = side * kLbD + depth, bid.z = shot.
+__device__ __forceinline__ void lb_canon_body(const FGrid &g, const FWave &w, const KModel &m, float sc,
+                                              float *__restrict__ buf, int E, const uint3 bid) {
+    const int bb = bid.x * blockDim.x + threadIdx.x;
+    if (bb >= E) return;
+    const int side = bid.y / kLbD, aa = bid.y % kLbD;
+    const long so = (long)bid.z * g.plane;
+    float *o = buf + (long)bid.z * lb_canon_floats(E) + ((long)side * kLbIn * kLbD + aa) * E + bb;
+    const bool in = bb < lb_side_len(g, side);
+#pragma unroll
+    for (int cc = 0; cc < kLbIn; ++cc) {
+        float v = 0.f;
+        if (in) {
+            int i, j;
+            float sg;
+            const int ch = lb_map(g, side, cc, aa, bb, i, j, sg);
+            v = sg * lb_input(g, w, m.q, so, ch, i, j, sc);
+        }
+        o[(long)cc * kLbD * E] = v;
+    }
+}
+__global__ __launch_bounds__(256) void k_lb_canon(FGrid g, FWave w, KModel m, float sc, float *__restrict__ buf, int E) {
+    lb_canon_body(g, w, m, sc, buf, E, blockIdx3());
+}
+
 // Network on the outer ring: for each ring cell, the side with the smallest
 // canonical depth (ties: x sides first, low before high), the 7x7x5 canonical
-// window, the composed filter, and the correction added to the increment:
-// D += delta, V_new = V + D.  Every window reads the increments before any
-// ring cell advances them (a window holds other ring cells), so the new
-// increments go to dnew[shot][ncells] and k_lb_apply stores them.  With
-// wts == NULL the ring is held at zero here (the pad without the network).
-__device__ __forceinline__ void lb_ring_body(const FGrid &g, const FWave &w, const KModel &m, const LbCell *__restrict__ cells,
-                                                 int ncells, const float *__restrict__ wts, int rb, float sc,
-                                                 float *__restrict__ dnew, const uint3 bid) {
+// window (from the staged rows), the composed filter (staged in LDS), and the
+// correction added to the increment: D += delta, V_new = V + D.  Every window
+// reads the increments before any ring cell advances them (a window holds
+// other ring cells), so the new increments go to dnew[shot][ncells] and
+// k_lb_apply stores them.  With wts == NULL the ring is held at zero here (the
+// pad without the network).
+__device__ __forceinline__ void lb_ring_body(const FGrid &g, const FWave &w, const LbCell *__restrict__ cells,
+                                             int ncells, const float *__restrict__ wts, int rb,
+                                             const float *__restrict__ canon, int E, float *__restrict__ dnew,
+                                             const uint3 bid) {
+    __shared__ float wl[kLbTaps];
+    if (wts) { // uniform: every lane of the block takes part
+        for (int t = threadIdx.x; t < kLbTaps; t += blockDim.x) wl[t] = wts[t];
+        __syncthreads();
+    }
     const int e = bid.x * blockDim.x + threadIdx.x;
     if (e >= ncells) return;
     const LbCell cl = cells[e];
@@ -188,17 +231,24 @@ __device__ __forceinline__ void lb_ring_body(const FGrid &g, const FWave &w, con
     const int a = da[side];
     const int b = side < 2 ? cl.j : cl.i;
     const int oc = side < 2 ? f : kLbT[f]; // canonical output channel of this field
+    const int len = lb_side_len(g, side);
+    const float *wo = wl + oc * kLbIn * kLbK * kLbK;
+    const float *cb = canon + (long)bid.z * lb_canon_floats(E) + (long)side * kLbIn * kLbD * E;
+    // taps in the order of the filter's layout; a staged value already carries
+    // the mirror sign, and w * (s x) equals (w s) x exactly for s = +-1
     float acc = 0.f;
-    const float *wo = wts + (long)oc * kLbIn * kLbK * kLbK;
+#pragma unroll
     for (int cc = 0; cc < kLbIn; ++cc) {
+#pragma unroll
         for (int dy = 0; dy < kLbK; ++dy) {
             const int aa = a + dy - 3;
             if (aa < 0) continue;
+            const float *row = cb + ((long)cc * kLbD + aa) * E;
+#pragma unroll
             for (int dx = 0; dx < kLbK; ++dx) {
-                int i, j;
-                float sg;
-                const int ch = lb_map(g, side, cc, aa, b + dx - 3, i, j, sg);
-                acc += wo[(cc * kLbK + dy) * kLbK + dx] * sg * lb_input(g, w, m.q, so, ch, i, j, sc);
+                const int bb = b + dx - 3;
+                const float x = (bb >= 0 && bb < len) ? row[bb] : 0.f;
+                acc += wo[(cc * kLbK + dy) * kLbK + dx] * x;
             }
         }
     }
@@ -208,10 +258,10 @@ __device__ __forceinline__ void lb_ring_body(const FGrid &g, const FWave &w, con
     if (side == 3) sg_out = kLbZS[f];
     dnew[(long)bid.z * ncells + e] = lb_gamma(g, 0) * (w.DV[f][k] + sg_out * acc);
 }
-__global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, KModel m, const LbCell *__restrict__ cells,
-                                                 int ncells, const float *__restrict__ wts, int rb, float sc,
-                                                 float *__restrict__ dnew) {
-    lb_ring_body(g, w, m, cells, ncells, wts, rb, sc, dnew, blockIdx3());
+__global__ __launch_bounds__(256) void k_lb_ring(FGrid g, FWave w, const LbCell *__restrict__ cells, int ncells,
+                                                 const float *__restrict__ wts, int rb, const float *__restrict__ canon,
+                                                 int E, float *__restrict__ dnew) {
+    lb_ring_body(g, w, cells, ncells, wts, rb, canon, E, dnew, blockIdx3());
 }
 
 // The ring's new increments from k_lb_ring: D = dnew, V_new = V + D.
