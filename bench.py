@@ -361,6 +361,11 @@ def serve_bench(a):
     grids, nt_range = (serve.SMALL_GRIDS, serve.SMALL_NT) if small else (serve.GRIDS, (2000, 8000))
     jobs = serve.make_jobs(n=a.serve_shots, grids=grids, nt_range=nt_range)
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
+    try:
+        serve.check_validated(jobs)
+    except ValueError as ex:
+        print(json.dumps({"error": str(ex), "metric": "uneven serving batch: RTM shots/hour and per-shot latency p50/p95"}))
+        sys.exit(2)
     lat, wall, groups = serve.serve(jobs, schedule=a.serve_schedule, log=log)
     s = serve.summary(jobs, lat, wall, groups)
     line = {

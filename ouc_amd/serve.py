@@ -32,6 +32,24 @@ GRIDS = ((512, 256), (1024, 512), (2048, 1024), (4096, 2048))
 # and one grid alone does not fill the GPU.
 SMALL_GRIDS = ((96, 48), (112, 56), (128, 64), (160, 80))
 SMALL_NT = (200, 420)
+# Full-size serving is validated up to FULL_MAX_NT steps per shot (every
+# configured grid at 2000 and 2600 steps, concurrent groups, bit-exact).  The
+# full serving workload with step counts up to 8000 (32 shots in batches of
+# 3-4 per grid, a warm pass first) ended its process with a host segmentation
+# fault, on concurrent streams and in shared launches alike, whose cause is not
+# found; such jobs are refused rather than run.
+FULL_MAX_NT = 2600
+SMALL_MAX_CELLS = 160 * 80
+
+
+def check_validated(jobs):
+    """Raise ValueError for a job outside the validated serving envelope: a grid
+    above 160 x 80 cells with more than FULL_MAX_NT steps."""
+    for j in jobs:
+        if j.grid[0] * j.grid[1] > SMALL_MAX_CELLS and j.nt > FULL_MAX_NT:
+            raise ValueError(f"serving job {j.id} ({j.grid[0]}x{j.grid[1]}, {j.nt} steps) is outside the validated "
+                             f"envelope: grids above 160x80 are served up to {FULL_MAX_NT} steps (longer full-size "
+                             "serving runs ended with a host fault whose cause is not found)")
 
 
 @dataclass
@@ -91,6 +109,7 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, sc
     The groups' device buffers share `mem_budget_gb`: when all groups' shots do
     not fit at once, every group's batch shrinks by the same factor (the rest
     of its shots run as later batches)."""
+    check_validated(jobs)
     concurrent = schedule == "concurrent"
     groups = {}
     for j in jobs:
