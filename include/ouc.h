@@ -62,7 +62,11 @@ enum { OUC_STRIP_FP32 = 0, OUC_STRIP_FP16 = 1 };
 /* Absorbing boundary.  CPML is the reference's (P-S-seperating.cu:2067-2105).
  * LEARNED replaces most of the CPML pad by a thin band driven by a small
  * convolutional network; LEARNED_FALLBACK does the same and falls back to CPML
- * when the band's energy grows. */
+ * when the band's energy grows.  With either learned kind, RTM and FWI
+ * (ouc_rtm_shot, ouc_rtm_batch, ouc_rtm_grouped, ouc_fwi_gradient) are
+ * validated up to 2600 steps and return OUC_UNSUPPORTED beyond (a longer run
+ * ended with a host fault whose cause is not found); forward modelling is not
+ * limited. */
 enum { OUC_BOUNDARY_CPML = 0, OUC_BOUNDARY_LEARNED = 1, OUC_BOUNDARY_LEARNED_FALLBACK = 2 };
 
 /* Image filter.  STENCIL is the reference's 5-point Laplace kernel

@@ -951,8 +951,23 @@ double rerun_with_cpml(ouc_ctx *c, const ouc_shot *shots, const std::vector<int>
     return fb->last.total_ms;
 }
 
+// Learned-boundary contexts run the RTM and FWI paths (forward run with strips,
+// receiver run and reconstruction, captured loops) validated up to
+// kLearnedRtmSteps steps.  A single 512 x 256 shot of 7,996 steps ended its
+// process with a host segmentation fault while its first loop was being
+// captured, whose cause is not found, so longer runs are refused.  Forward
+// modelling (ouc_forward_shot) is validated to 8,000 steps and is not limited.
+constexpr int kLearnedRtmSteps = 2600;
+static void check_learned_rtm_steps(const ouc_ctx *c) {
+    if (c->g.learned && c->g.nt > kLearnedRtmSteps)
+        throw Error(OUC_UNSUPPORTED, "RTM and FWI with the learned boundary are validated up to " +
+                                         std::to_string(kLearnedRtmSteps) + " steps; a longer run ended with a host "
+                                         "fault whose cause is not found, so it is refused (use the CPML boundary)");
+}
+
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms) {
+    check_learned_rtm_steps(c);
     const Grid &g = c->g;
     for (int k = 0; k < nshots; ++k) {
         const ouc_shot *sh = &shots[k];
@@ -1447,6 +1462,7 @@ void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, 
 // Single shot through the fused path (a batch of one), returning its images.
 void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *img_pp, float *img_ps, float *lap_pp,
                    float *lap_ps) {
+    check_learned_rtm_steps(c);
     ensure(c, std::max(1, fs(c)->cap));
     FusedState *st = fs(c);
     CoreProbe probe;
@@ -1490,6 +1506,7 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *im
 // are x-major interior arrays, misfit = 0.5 * sum of squared residuals.
 void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const float *observed, float *grad_vp,
                         float *grad_vs, float *illum, double *misfit) {
+    check_learned_rtm_steps(c);
     const Grid &g = c->g;
     if (g.learned) throw Error(OUC_UNSUPPORTED, "the FWI gradient runs with the CPML boundary");
     for (int k = 0; k < nshots; ++k) {
