@@ -56,6 +56,8 @@ def parse():
                          "learned boundary with CPML fallback, FP16 strips, autotuned tiles")
     ap.add_argument("--serve-shots", type=int, default=32)
     ap.add_argument("--serve-schedule", choices=["sjf", "concurrent", "grouped"], default="sjf")
+    ap.add_argument("--serve-boundary", choices=["learned", "cpml"], default="learned",
+                    help="learned: 8-cell learned band with CPML fallback; cpml: 30-cell CPML")
     ap.add_argument("--serve-grids", choices=["full", "small"], default="full",
                     help="full: 512x256..4096x2048, nt 2000-8000; small: 96x48..160x80, nt 200-420 "
                          "(the grouped launch's validated envelope)")
@@ -361,12 +363,13 @@ def serve_bench(a):
     grids, nt_range = (serve.SMALL_GRIDS, serve.SMALL_NT) if small else (serve.GRIDS, (2000, 8000))
     jobs = serve.make_jobs(n=a.serve_shots, grids=grids, nt_range=nt_range)
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
+    learned = a.serve_boundary == "learned"
     try:
-        serve.check_validated(jobs)
+        serve.check_validated(jobs, learned)
     except ValueError as ex:
         print(json.dumps({"error": str(ex), "metric": "uneven serving batch: RTM shots/hour and per-shot latency p50/p95"}))
         sys.exit(2)
-    lat, wall, groups = serve.serve(jobs, schedule=a.serve_schedule, log=log)
+    lat, wall, groups = serve.serve(jobs, learned=learned, schedule=a.serve_schedule, log=log)
     s = serve.summary(jobs, lat, wall, groups)
     line = {
         "metric": "uneven serving batch: RTM shots/hour and per-shot latency p50/p95",
@@ -375,7 +378,8 @@ def serve_bench(a):
         "dtype": "f32", "data": "synthetic (seeded layered models per grid, random shots)",
         "config": {"workload": f"{len(jobs)} RTM shots, grids {'/'.join(f'{g[0]}x{g[1]}' for g in grids)}, "
                                f"nt {nt_range[0]}-{nt_range[1]}, "
-                               "learned 8-cell boundary with CPML fallback, FP16 strips, autotuned tiles, "
+                               + ("learned 8-cell boundary with CPML fallback" if learned else "30-cell CPML")
+                               + ", FP16 strips, autotuned tiles, "
                                + ("all grids in shared launches per time step" if a.serve_schedule == "grouped"
                                   else f"per-grid batches, schedule {a.serve_schedule}")},
         "latency_ms_p50": s["latency_ms_p50"], "latency_ms_p95": s["latency_ms_p95"], "groups": s["groups"],

@@ -32,19 +32,21 @@ GRIDS = ((512, 256), (1024, 512), (2048, 1024), (4096, 2048))
 # and one grid alone does not fill the GPU.
 SMALL_GRIDS = ((96, 48), (112, 56), (128, 64), (160, 80))
 SMALL_NT = (200, 420)
-# Full-size serving is validated up to FULL_MAX_NT steps per shot (every
-# configured grid at 2000 and 2600 steps, concurrent groups, bit-exact).  The
-# full serving workload with step counts up to 8000 (32 shots in batches of
-# 3-4 per grid, a warm pass first) ended its process with a host segmentation
-# fault, on concurrent streams and in shared launches alike, whose cause is not
-# found; such jobs are refused rather than run.
+# Full-size serving on the learned boundary is validated up to FULL_MAX_NT
+# steps per shot (every configured grid at 2000 and 2600 steps, concurrent
+# groups, bit-exact); the engine refuses longer learned-boundary RTM runs (one
+# of 7996 steps ended its process with a host fault whose cause is not found),
+# and so does the serving driver, before any work.
 FULL_MAX_NT = 2600
 SMALL_MAX_CELLS = 160 * 80
 
 
-def check_validated(jobs):
-    """Raise ValueError for a job outside the validated serving envelope: a grid
-    above 160 x 80 cells with more than FULL_MAX_NT steps."""
+def check_validated(jobs, learned=True):
+    """Raise ValueError for a job outside the validated serving envelope: with
+    the learned boundary, a grid above 160 x 80 cells with more than
+    FULL_MAX_NT steps (CPML groups are not limited)."""
+    if not learned:
+        return
     for j in jobs:
         if j.grid[0] * j.grid[1] > SMALL_MAX_CELLS and j.nt > FULL_MAX_NT:
             raise ValueError(f"serving job {j.id} ({j.grid[0]}x{j.grid[1]}, {j.nt} steps) is outside the validated "
@@ -109,7 +111,7 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, sc
     The groups' device buffers share `mem_budget_gb`: when all groups' shots do
     not fit at once, every group's batch shrinks by the same factor (the rest
     of its shots run as later batches)."""
-    check_validated(jobs)
+    check_validated(jobs, learned)
     concurrent = schedule == "concurrent"
     groups = {}
     for j in jobs:
