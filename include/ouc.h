@@ -195,10 +195,11 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
  * what ouc_rtm_batch gives for its own shots.  stack_pp_lap[i], stack_ps_lap[i]
  * receive context i's stacks (either array, or an entry, may be NULL).  The
  * contexts must share the device, half-order (5 or 6), boundary kind, strip
- * precision and semantics; at most 8 contexts; each context at most 420
- * steps and 160 x 80 interior cells (the validated envelope: a larger group
- * ended its process with a host fault not yet explained; such groups return
- * OUC_UNSUPPORTED before any work).  With LEARNED_FALLBACK,
+ * precision and semantics; at most 8 contexts.  With a learned boundary each
+ * context holds at most 420 steps and 160 x 80 interior cells (the validated
+ * envelope: a larger learned group ended its process with a host fault not yet
+ * explained; such groups return OUC_UNSUPPORTED before any work); CPML groups
+ * are not limited.  With LEARNED_FALLBACK,
  * diverged shots run again on CPML and the call returns
  * OUC_NUMERIC_FALLBACK with valid stacks.  No counterpart in the reference,
  * whose shots all share one grid (the shot loop, P-S-seperating.cu:2108-2401,

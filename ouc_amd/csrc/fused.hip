@@ -1370,17 +1370,19 @@ void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, 
         }
     }
     if (a->g.N != 5 && a->g.N != 6) throw Error(OUC_UNSUPPORTED, "grouped runs support half-orders 5 and 6");
-    // Grouped runs are validated (bit-identical to each grid's own batch) up to
-    // kGroupMaxSteps steps and kGroupMaxCells interior cells per grid.  A larger
-    // group (7,996 steps, grids to 4096 x 2048) ended its process with a host
-    // segmentation fault whose cause is not found yet, so larger groups are
-    // refused; run their grids with ouc_rtm_batch.
+    // Learned-boundary groups are validated (bit-identical to each grid's own
+    // batch) up to kGroupMaxSteps steps and kGroupMaxCells interior cells per
+    // grid.  A larger learned group (7,996 steps, grids to 4096 x 2048) ended its
+    // process with a host segmentation fault whose cause is not found, as a
+    // single learned context of 7,996 steps later did, so larger learned groups
+    // are refused.  CPML groups are not limited.
     constexpr int kGroupMaxSteps = 420;
     constexpr long kGroupMaxCells = 160L * 80L;
     for (int i = 0; i < n; ++i)
-        if (cs[i]->g.nt > kGroupMaxSteps || (long)cs[i]->g.nx * cs[i]->g.nz > kGroupMaxCells)
-            throw Error(OUC_UNSUPPORTED, "grouped runs are validated up to " + std::to_string(kGroupMaxSteps) +
-                                             " steps and " + std::to_string(kGroupMaxCells) +
+        if (a->g.learned && (cs[i]->g.nt > kGroupMaxSteps || (long)cs[i]->g.nx * cs[i]->g.nz > kGroupMaxCells))
+            throw Error(OUC_UNSUPPORTED, "grouped runs on the learned boundary are validated up to " +
+                                             std::to_string(kGroupMaxSteps) + " steps and " +
+                                             std::to_string(kGroupMaxCells) +
                                              " interior cells per grid; a larger group ended with a host fault "
                                              "whose cause is not found, so it is refused (use ouc_rtm_batch)");
     const std::vector<int> cap = group_caps(cs, std::vector<int>(nshots, nshots + n));

@@ -125,16 +125,51 @@ def test_grouped_refuses_mixed_contexts():
             rtm_grouped([a, a], [shots, shots])
 
 
-def test_grouped_refuses_groups_beyond_the_validated_envelope():
-    """Groups past 420 steps or 160 x 80 interior cells are refused before any
-    work (a larger group ended its process with a host fault whose cause is
-    not found)."""
-    print("[case] grouped: 200x100 grid and a 421-step grid refused", flush=True)
+def test_grouped_refuses_learned_groups_beyond_the_validated_envelope():
+    """Learned-boundary groups past 420 steps or 160 x 80 interior cells are
+    refused before any work (a larger learned group ended its process with a
+    host fault whose cause is not found)."""
+    print("[case] grouped: learned 200x100 grid and a learned 421-step grid refused", flush=True)
+    from ouc_amd.learned import load_weights
+
     vp, vs, _ = velocity_models.layered_random(200, 100, seed=4)
-    big = abi.make_params(200, 100, 20, 5, 300, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED)
+    big = abi.make_params(200, 100, 30, 5, 300, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED,
+                          boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
     vp2, vs2, _ = velocity_models.layered_random(96, 48, seed=4)
-    long_ = abi.make_params(96, 48, 20, 5, 421, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED)
+    long_ = abi.make_params(96, 48, 30, 5, 421, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED,
+                            boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
     with Engine(big, vp, vs) as a, Engine(long_, vp2, vs2) as b:
         for e in (a, b):
+            e.set_learned_boundary(load_weights())
             with pytest.raises(OucError, match="UNSUPPORTED"):
                 rtm_grouped([e], [[dict(sx=40, sz=0, rec_z=0, fm=22.0)]])
+
+
+LARGE = [(512, 256, 1000), (1024, 512, 800), (768, 384, 700)]
+
+
+def test_grouped_cpml_at_larger_sizes_equals_each_grid_alone():
+    """CPML groups are not limited to the learned envelope: three grids up to
+    1024 x 512 and 1000 steps, FP16 strips, each grid's stack bit for bit its
+    own batch's."""
+    print(f"[case] grouped CPML grids {LARGE} fp16 strips", flush=True)
+    es = []
+    try:
+        for k, (nx, nz, nt) in enumerate(LARGE):
+            vp, vs, _ = velocity_models.layered_random(nx, nz, seed=40 + k)
+            p = abi.make_params(nx, nz, 30, 5, nt, 5.0, 5.0, 4e-4, R=1e-3, strip_precision=abi.STRIP_FP16,
+                                engine=abi.ENGINE_FUSED)
+            es.append(Engine(p, vp, vs))
+        lists = []
+        for k, (nx, nz, nt) in enumerate(LARGE):
+            sh = [dict(sx=x, sz=0, rec_z=0, fm=20.0, mute=1) for x in (nx // 4, 3 * nx // 4)]
+            sh[-1]["nt"] = nt - 100 * (k + 1)
+            lists.append(sh)
+        grouped = rtm_grouped(es, lists)
+        for k, (e, shots) in enumerate(zip(es, lists)):
+            own = e.rtm_batch(shots)
+            for key in ("stack_pp", "stack_ps"):
+                assert grouped[k][key].tobytes() == own[key].tobytes(), (LARGE[k], key)
+    finally:
+        for e in es:
+            e.close()
