@@ -115,14 +115,14 @@ struct FusedState {
     // wavelets and start steps travel through device memory).
     struct LoopGraph {
         int B = -1, rtm = -1, ncomp = -1, zr = -1;
-        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraphExec_t> execs; // one per chunk of kGraphSteps steps, replayed in order
         void drop() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            exec = nullptr;
+            for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+            execs.clear();
             B = -1;
         }
         bool matches(int b, int r, int n, int z) const {
-            return exec && B == b && rtm == r && ncomp == n && zr == z;
+            return !execs.empty() && B == b && rtm == r && ncomp == n && zr == z;
         }
     } gfwd, gbwd;
     void drop_graphs() { gfwd.drop(); gbwd.drop(); }
@@ -146,34 +146,55 @@ struct FusedState {
     }
 };
 
-// Enqueues `body` on stream s directly, or replays the graph it was captured
-// into the first time for this key (HIP stream capture).
+// Time loops are captured in chunks of at most kGraphSteps steps, one graph
+// each, replayed in order on the same stream.  A loop captured whole is one
+// dependency chain of up to ten launches per step; runs whose captured chain
+// exceeded about 74,000 launches (a learned-boundary RTM loop of 7,996 steps:
+// 81,000) ended their process with a host segmentation fault while the graph
+// was built, and every run below that (73,000 and fewer) completed.  Chunks
+// bound the chain at about 21,000 launches; the kernels and their order are
+// those of the whole loop.
+constexpr int kGraphSteps = 2048;
+
+// Enqueues body(i0, i1) -- iterations [i0, i1) of an nsteps-iteration loop --
+// on stream s directly, or replays the graphs it was captured into, chunk by
+// chunk, the first time for this key (HIP stream capture).  body carries the
+// loop's running state across calls, so the chunks are captured in order.
 template <class F>
-void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int rtm, int ncomp, int zr, F &&body) {
+void run_loop(hipStream_t s, bool graph, FusedState::LoopGraph &lg, int B, int rtm, int ncomp, int zr, int nsteps,
+              F &&body) {
     if (!graph) {
-        body();
+        body(0, nsteps);
         return;
     }
     if (!lg.matches(B, rtm, ncomp, zr)) {
         lg.drop();
-        hipGraph_t gr = nullptr;
-        OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            body();
+            for (int i0 = 0; i0 < nsteps; i0 += kGraphSteps) {
+                const int i1 = std::min(nsteps, i0 + kGraphSteps);
+                hipGraph_t gr = nullptr;
+                OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    body(i0, i1);
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &gr);
+                    if (gr) (void)hipGraphDestroy(gr);
+                    throw;
+                }
+                OUC_HIP(hipStreamEndCapture(s, &gr));
+                hipGraphExec_t ex = nullptr;
+                hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                OUC_HIP(e);
+                lg.execs.push_back(ex);
+            }
         } catch (...) {
-            (void)hipStreamEndCapture(s, &gr);
-            if (gr) (void)hipGraphDestroy(gr);
+            lg.drop();
             throw;
         }
-        OUC_HIP(hipStreamEndCapture(s, &gr));
-        hipGraphExec_t ex = nullptr;
-        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        OUC_HIP(e);
-        lg.exec = ex;
         lg.B = B; lg.rtm = rtm; lg.ncomp = ncomp; lg.zr = zr;
     }
-    OUC_HIP(hipGraphLaunch(lg.exec, s));
+    for (hipGraphExec_t ex : lg.execs) OUC_HIP(hipGraphLaunch(ex, s));
 }
 
 constexpr int kMonStride = 8; // steps between boundary-monitor launches
@@ -702,8 +723,8 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     const bool graph_b = c->graphs && !snap && !rsnap;
     if (graph_b) st->ensure_side(); // the captured backward loop runs independent work on two streams
     int rb = 0, ksnap = 0;
-    run_loop(s, graph, st->gfwd, B, rtm, ncomp, zr, [&] {
-    for (int t = 0; t < nt; ++t) {
+    run_loop(s, graph, st->gfwd, B, rtm, ncomp, zr, nt, [&](int i0, int i1) {
+    for (int t = i0; t < i1; ++t) {
         a.t = t;
         a.rb = rb;
         wave_step_probed(c, st, L, st->S, a, probe, nlaunch, mon_f, rtm);
@@ -772,8 +793,9 @@ void run_batch(ouc_ctx *c, const ouc_shot *shots, int B, bool rtm, int ncomp, Co
     // stream: iteration t decodes slot t, and the reconstruction of step t reads
     // slots t+1 and (EXACT) t
     const dim3 dgrid(rblocks, RS_NF, B);
-    run_loop(s, graph_b, st->gbwd, B, fwi ? 2 : 1, ncomp, zr, [&] {
-    for (int t = nt - 1; t >= 0; --t) {
+    run_loop(s, graph_b, st->gbwd, B, fwi ? 2 : 1, ncomp, zr, nt, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+        const int t = nt - 1 - i; // the backward loop runs t = nt-1 .. 0
         if (graph_b) {
             OUC_HIP(hipEventRecord(st->evf, s));
             OUC_HIP(hipStreamWaitEvent(sr, st->evf, 0));
@@ -1050,11 +1072,12 @@ namespace {
 struct GroupGraphs {
     std::vector<unsigned char> key; // the context table and loop shape the graphs were captured for
     DevBuf tab;                     // device copy of the context table
-    hipGraphExec_t fwd = nullptr, bwd = nullptr;
+    std::vector<hipGraphExec_t> fwd, bwd; // chunks of kGraphSteps steps, replayed in order
     void drop() {
-        if (fwd) (void)hipGraphExecDestroy(fwd);
-        if (bwd) (void)hipGraphExecDestroy(bwd);
-        fwd = bwd = nullptr;
+        for (hipGraphExec_t e : fwd) (void)hipGraphExecDestroy(e);
+        for (hipGraphExec_t e : bwd) (void)hipGraphExecDestroy(e);
+        fwd.clear();
+        bwd.clear();
     }
     ~GroupGraphs() { drop(); }
 };
@@ -1243,31 +1266,44 @@ void run_group_batch(const std::vector<ouc_ctx *> &cs, const std::vector<const o
             if (nb) k_lb_monitor_grp<RCV><<<nb, dim3(64, 4), 0, q>>>(tab, Lg);
         }
     };
-    auto run = [&](hipGraphExec_t &ex, auto &&body) {
+    // body(i0, i1) runs iterations [i0, i1) of the T-iteration loop; captured
+    // in chunks of kGraphSteps iterations like run_loop's
+    auto run = [&](std::vector<hipGraphExec_t> &execs, auto &&body) {
         if (!graph) {
-            body();
+            body(0, T);
             return;
         }
-        if (!ex) {
-            hipGraph_t gr = nullptr;
-            OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        if (execs.empty()) {
             try {
-                body();
+                for (int i0 = 0; i0 < T; i0 += kGraphSteps) {
+                    const int i1 = std::min(T, i0 + kGraphSteps);
+                    hipGraph_t gr = nullptr;
+                    OUC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        body(i0, i1);
+                    } catch (...) {
+                        (void)hipStreamEndCapture(s, &gr);
+                        if (gr) (void)hipGraphDestroy(gr);
+                        throw;
+                    }
+                    OUC_HIP(hipStreamEndCapture(s, &gr));
+                    hipGraphExec_t ex = nullptr;
+                    hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(gr);
+                    OUC_HIP(e);
+                    execs.push_back(ex);
+                }
             } catch (...) {
-                (void)hipStreamEndCapture(s, &gr);
-                if (gr) (void)hipGraphDestroy(gr);
+                for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+                execs.clear();
                 throw;
             }
-            OUC_HIP(hipStreamEndCapture(s, &gr));
-            hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(gr);
-            OUC_HIP(e);
         }
-        OUC_HIP(hipGraphLaunch(ex, s));
+        for (hipGraphExec_t ex : execs) OUC_HIP(hipGraphLaunch(ex, s));
     };
     OUC_HIP(hipEventRecord(lead->ev[4], s));
-    run(G.fwd, [&] {
-        for (int tau = 0; tau < T; ++tau) {
+    run(G.fwd, [&](int i0, int i1) {
+        for (int tau = i0; tau < i1; ++tau) {
             wave_step(tau, std::false_type{}, s);
             int nb;
             const GLaunch Lg = mk(tau, [&](int i, int) { return hc[i].rblocks * hc[i].B; }, nb);
@@ -1286,8 +1322,9 @@ void run_group_batch(const std::vector<ouc_ctx *> &cs, const std::vector<const o
     if (graph) st0->ensure_side();
     hipStream_t sr = graph ? st0->s2 : s;
     OUC_HIP(hipEventRecord(lead->ev[6], s));
-    run(G.bwd, [&] {
-        for (int tau = T - 1; tau >= 0; --tau) {
+    run(G.bwd, [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const int tau = T - 1 - i; // backward: tau = T-1 .. 0
             if (graph) {
                 OUC_HIP(hipEventRecord(st0->evf, s));
                 OUC_HIP(hipStreamWaitEvent(sr, st0->evf, 0));
