@@ -62,11 +62,7 @@ enum { OUC_STRIP_FP32 = 0, OUC_STRIP_FP16 = 1 };
 /* Absorbing boundary.  CPML is the reference's (P-S-seperating.cu:2067-2105).
  * LEARNED replaces most of the CPML pad by a thin band driven by a small
  * convolutional network; LEARNED_FALLBACK does the same and falls back to CPML
- * when the band's energy grows.  With either learned kind, RTM and FWI
- * (ouc_rtm_shot, ouc_rtm_batch, ouc_rtm_grouped, ouc_fwi_gradient) are
- * validated up to 2600 steps and return OUC_UNSUPPORTED beyond (a longer run
- * ended with a host fault whose cause is not found); forward modelling is not
- * limited. */
+ * when the band's energy grows. */
 enum { OUC_BOUNDARY_CPML = 0, OUC_BOUNDARY_LEARNED = 1, OUC_BOUNDARY_LEARNED_FALLBACK = 2 };
 
 /* Image filter.  STENCIL is the reference's 5-point Laplace kernel
@@ -195,11 +191,7 @@ ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
  * what ouc_rtm_batch gives for its own shots.  stack_pp_lap[i], stack_ps_lap[i]
  * receive context i's stacks (either array, or an entry, may be NULL).  The
  * contexts must share the device, half-order (5 or 6), boundary kind, strip
- * precision and semantics; at most 8 contexts.  With a learned boundary each
- * context holds at most 420 steps and 160 x 80 interior cells (the validated
- * envelope: a larger learned group ended its process with a host fault not yet
- * explained; such groups return OUC_UNSUPPORTED before any work); CPML groups
- * are not limited.  With LEARNED_FALLBACK,
+ * precision and semantics; at most 8 contexts.  With LEARNED_FALLBACK,
  * diverged shots run again on CPML and the call returns
  * OUC_NUMERIC_FALLBACK with valid stacks.  No counterpart in the reference,
  * whose shots all share one grid (the shot loop, P-S-seperating.cu:2108-2401,

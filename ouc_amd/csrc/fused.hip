@@ -973,23 +973,8 @@ double rerun_with_cpml(ouc_ctx *c, const ouc_shot *shots, const std::vector<int>
     return fb->last.total_ms;
 }
 
-// Learned-boundary contexts run the RTM and FWI paths (forward run with strips,
-// receiver run and reconstruction, captured loops) validated up to
-// kLearnedRtmSteps steps.  A single 512 x 256 shot of 7,996 steps ended its
-// process with a host segmentation fault while its first loop was being
-// captured, whose cause is not found, so longer runs are refused.  Forward
-// modelling (ouc_forward_shot) is validated to 8,000 steps and is not limited.
-constexpr int kLearnedRtmSteps = 2600;
-static void check_learned_rtm_steps(const ouc_ctx *c) {
-    if (c->g.learned && c->g.nt > kLearnedRtmSteps)
-        throw Error(OUC_UNSUPPORTED, "RTM and FWI with the learned boundary are validated up to " +
-                                         std::to_string(kLearnedRtmSteps) + " steps; a longer run ended with a host "
-                                         "fault whose cause is not found, so it is refused (use the CPML boundary)");
-}
-
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
                      float *per_shot_ms) {
-    check_learned_rtm_steps(c);
     const Grid &g = c->g;
     for (int k = 0; k < nshots; ++k) {
         const ouc_shot *sh = &shots[k];
@@ -1407,21 +1392,6 @@ void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, 
         }
     }
     if (a->g.N != 5 && a->g.N != 6) throw Error(OUC_UNSUPPORTED, "grouped runs support half-orders 5 and 6");
-    // Learned-boundary groups are validated (bit-identical to each grid's own
-    // batch) up to kGroupMaxSteps steps and kGroupMaxCells interior cells per
-    // grid.  A larger learned group (7,996 steps, grids to 4096 x 2048) ended its
-    // process with a host segmentation fault whose cause is not found, as a
-    // single learned context of 7,996 steps later did, so larger learned groups
-    // are refused.  CPML groups are not limited.
-    constexpr int kGroupMaxSteps = 420;
-    constexpr long kGroupMaxCells = 160L * 80L;
-    for (int i = 0; i < n; ++i)
-        if (a->g.learned && (cs[i]->g.nt > kGroupMaxSteps || (long)cs[i]->g.nx * cs[i]->g.nz > kGroupMaxCells))
-            throw Error(OUC_UNSUPPORTED, "grouped runs on the learned boundary are validated up to " +
-                                             std::to_string(kGroupMaxSteps) + " steps and " +
-                                             std::to_string(kGroupMaxCells) +
-                                             " interior cells per grid; a larger group ended with a host fault "
-                                             "whose cause is not found, so it is refused (use ouc_rtm_batch)");
     const std::vector<int> cap = group_caps(cs, std::vector<int>(nshots, nshots + n));
     for (int i = 0; i < n; ++i) ensure(cs[i], std::max(1, cap[i]));
     std::vector<DevBuf> stack(n);
@@ -1501,7 +1471,6 @@ void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, 
 // Single shot through the fused path (a batch of one), returning its images.
 void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *img_pp, float *img_ps, float *lap_pp,
                    float *lap_ps) {
-    check_learned_rtm_steps(c);
     ensure(c, std::max(1, fs(c)->cap));
     FusedState *st = fs(c);
     CoreProbe probe;
@@ -1545,7 +1514,6 @@ void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *im
 // are x-major interior arrays, misfit = 0.5 * sum of squared residuals.
 void fused_fwi_gradient(ouc_ctx *c, const ouc_shot *shots, int nshots, const float *observed, float *grad_vp,
                         float *grad_vs, float *illum, double *misfit) {
-    check_learned_rtm_steps(c);
     const Grid &g = c->g;
     if (g.learned) throw Error(OUC_UNSUPPORTED, "the FWI gradient runs with the CPML boundary");
     for (int k = 0; k < nshots; ++k) {

@@ -27,33 +27,10 @@ from .engine import Engine, rtm_grouped
 from .velocity_models import layered_random
 
 GRIDS = ((512, 256), (1024, 512), (2048, 1024), (4096, 2048))
-# Small grids and step counts inside the grouped launch's validated envelope
-# (at most 160 x 80 cells and 420 steps per grid): where launches are short
-# and one grid alone does not fill the GPU.
+# Small grids and short runs: launches are short and one grid alone does not
+# fill the GPU, where shared launches over all grids pay most.
 SMALL_GRIDS = ((96, 48), (112, 56), (128, 64), (160, 80))
 SMALL_NT = (200, 420)
-# Full-size serving on the learned boundary is validated up to FULL_MAX_NT
-# steps per shot (every configured grid at 2000 and 2600 steps, concurrent
-# groups, bit-exact); the engine refuses longer learned-boundary RTM runs (one
-# of 7996 steps ended its process with a host fault whose cause is not found),
-# and so does the serving driver, before any work.
-FULL_MAX_NT = 2600
-SMALL_MAX_CELLS = 160 * 80
-
-
-def check_validated(jobs, learned=True):
-    """Raise ValueError for a job outside the validated serving envelope: with
-    the learned boundary, a grid above 160 x 80 cells with more than
-    FULL_MAX_NT steps (CPML groups are not limited)."""
-    if not learned:
-        return
-    for j in jobs:
-        if j.grid[0] * j.grid[1] > SMALL_MAX_CELLS and j.nt > FULL_MAX_NT:
-            raise ValueError(f"serving job {j.id} ({j.grid[0]}x{j.grid[1]}, {j.nt} steps) is outside the validated "
-                             f"envelope: grids above 160x80 are served up to {FULL_MAX_NT} steps (longer full-size "
-                             "serving runs ended with a host fault whose cause is not found)")
-
-
 @dataclass
 class Job:
     grid: tuple
@@ -105,13 +82,10 @@ def serve(jobs, learned=True, strips=abi.STRIP_FP16, weights=None, tune=True, sc
     shots) first -- small grids finish early instead of queueing behind large
     kernels; "concurrent": all groups at once, one host thread and stream each;
     "grouped": all groups in shared launches (one launch per kernel and time
-    step over every grid's shots), a shot's latency being the end of the call;
-    the grouped launch refuses grids beyond 420 steps or 160 x 80 cells (see
-    ouc_rtm_grouped).
+    step over every grid's shots), a shot's latency being the end of the call.
     The groups' device buffers share `mem_budget_gb`: when all groups' shots do
     not fit at once, every group's batch shrinks by the same factor (the rest
     of its shots run as later batches)."""
-    check_validated(jobs, learned)
     concurrent = schedule == "concurrent"
     groups = {}
     for j in jobs:

@@ -113,24 +113,3 @@ def test_grouped_run_falls_back_to_cpml(bad_weights):
         for e in engines:
             e.close()
 
-
-def test_long_learned_rtm_is_refused():
-    """RTM and FWI on a learned-boundary context beyond the validated 2600
-    steps return UNSUPPORTED before any work; forward modelling is not limited."""
-    from ouc_amd.engine import OucError
-    from ouc_amd.learned import load_weights
-
-    grid = (96, 48)
-    print(f"[case] learned RTM at 2601 steps refused on {grid}", flush=True)
-    vp, vs = serve.model_for(grid, 11)
-    with Engine(_params(grid, 2601, abi.BOUNDARY_LEARNED_FALLBACK), vp, vs) as e:
-        e.set_learned_boundary(load_weights())
-        shot = dict(sx=40, sz=0, rec_z=0, fm=20.0, mute=1)
-        with pytest.raises(OucError, match="UNSUPPORTED"):
-            e.rtm_batch([shot])
-        with pytest.raises(OucError, match="UNSUPPORTED"):
-            e.rtm_shot(shot)
-        with pytest.raises(OucError, match="UNSUPPORTED"):
-            e.fwi_gradient([shot], np.zeros((2, grid[0], 2601), np.float32))
-        seis = e.forward_shot(dict(sx=40, sz=0, rec_z=0, fm=20.0))
-        assert np.isfinite(seis).all()

@@ -125,24 +125,32 @@ def test_grouped_refuses_mixed_contexts():
             rtm_grouped([a, a], [shots, shots])
 
 
-def test_grouped_refuses_learned_groups_beyond_the_validated_envelope():
-    """Learned-boundary groups past 420 steps or 160 x 80 interior cells are
-    refused before any work (a larger learned group ended its process with a
-    host fault whose cause is not found)."""
-    print("[case] grouped: learned 200x100 grid and a learned 421-step grid refused", flush=True)
+def test_grouped_learned_groups_past_the_small_sizes_equal_each_grid_alone():
+    """Learned-boundary groups of a 200 x 100 grid and a 96 x 48 grid of 700
+    steps: each grid's stack bit for bit its own batch's."""
+    print("[case] grouped learned: 200x100 and 96x48 grids, 700 steps", flush=True)
     from ouc_amd.learned import load_weights
 
-    vp, vs, _ = velocity_models.layered_random(200, 100, seed=4)
-    big = abi.make_params(200, 100, 30, 5, 300, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED,
-                          boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
-    vp2, vs2, _ = velocity_models.layered_random(96, 48, seed=4)
-    long_ = abi.make_params(96, 48, 30, 5, 421, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED,
-                            boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
-    with Engine(big, vp, vs) as a, Engine(long_, vp2, vs2) as b:
-        for e in (a, b):
+    es = []
+    try:
+        for k, (nx, nz) in enumerate(((200, 100), (96, 48))):
+            vp, vs, _ = velocity_models.layered_random(nx, nz, seed=4 + k)
+            p = abi.make_params(nx, nz, 30, 5, 700, 5.0, 5.0, 4e-4, engine=abi.ENGINE_FUSED,
+                                strip_precision=abi.STRIP_FP16, boundary=abi.BOUNDARY_LEARNED_FALLBACK, band=8)
+            e = Engine(p, vp, vs)
             e.set_learned_boundary(load_weights())
-            with pytest.raises(OucError, match="UNSUPPORTED"):
-                rtm_grouped([e], [[dict(sx=40, sz=0, rec_z=0, fm=22.0)]])
+            es.append(e)
+        lists = [[dict(sx=40, sz=0, rec_z=0, fm=22.0, mute=1), dict(sx=70, sz=0, rec_z=0, fm=22.0, mute=1, nt=600)]
+                 for _ in es]
+        grouped = rtm_grouped(es, lists)
+        for k, (e, shots) in enumerate(zip(es, lists)):
+            own = e.rtm_batch(shots)
+            for key in ("stack_pp", "stack_ps"):
+                assert grouped[k][key].tobytes() == own[key].tobytes(), (k, key)
+        assert not getattr(es[0], "fell_back", False)
+    finally:
+        for e in es:
+            e.close()
 
 
 LARGE = [(512, 256, 1000), (1024, 512, 800), (768, 384, 700)]
