@@ -7,10 +7,13 @@ x = 16 + 32 s, boundary strips in scaled FP16 (FP32 compute).  A "step" is one
 batch of --batch shots (8 by default: the 64-shot layout runs as eight such
 batches) through the full RTM path (forward with strip saving, mute, backward
 with reconstruction, receiver propagation and imaging, Laplacian, stack).
-Inputs are resident on the GPU before the timed region; value = shots per
-hour of the whole job.  A wall-clock guard (--budget-s) stops the warmup and
-timed loops early when the next step and the legs after it would not fit, and
-the line reports the steps that ran.
+With --contexts C (3 by default) the job's batches run C at a time, each on
+its own context and stream (ouc_rtm_concurrent), and the stack is still summed
+in shot order.  Inputs are resident on the GPU before the timed region; value
+= shots per hour of the whole job over the timed region's wall time.  A
+wall-clock guard (--budget-s) shortens the warmup and the timed region when
+they and the legs after them would not fit, and the line reports the steps
+that ran.
 
 Also reported: Gcell-updates/s, canonical and own-bytes HBM rates, the plain
 (reference kernel structure, one shot at a time, FP32 strips) shots/hour on a
@@ -39,6 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0,
                     help="shots per step (0: 8 fused, 1 plain)")
+    ap.add_argument("--contexts", type=int, default=3,
+                    help="fused engine: contexts (one stream and host thread each) that run the job's batches at "
+                         "once through ouc_rtm_concurrent; 1 runs them one after another")
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--engine", choices=["fused", "plain"], default="fused")
     ap.add_argument("--strips", choices=["fp16", "fp32"], default="fp16")
@@ -139,101 +145,146 @@ def _elapsed():
 
 def rtm_bench(a):
     from ouc_amd import abi, bytes_model, velocity_models as vm
-    from ouc_amd.engine import Engine
+    from ouc_amd.engine import Engine, rtm_concurrent
 
     engine = abi.ENGINE_FUSED if a.engine == "fused" else abi.ENGINE_PLAIN
     strips = abi.STRIP_FP16 if a.strips == "fp16" else abi.STRIP_FP32
     p, vp, vs, shots = vm.cfg_rtm_2048(nt=a.nt, engine=engine, strip_precision=strips)
     batch = a.batch or (8 if engine == abi.ENGINE_FUSED else 1)
-    nsteps_total = a.warmup + a.steps
+    nctx = max(1, a.contexts) if engine == abi.ENGINE_FUSED else 1
+    nsteps_total = a.warmup + a.steps + 1  # + the solo leg's batch
     if nsteps_total * batch > len(shots):
         shots = shots * ((nsteps_total * batch) // len(shots) + 1)
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
-    log(f"{a.engine} engine, batch {batch}, {a.steps} timed + {a.warmup} warmup steps, grid {p.nx}x{p.nz} "
-        f"nt={p.nt}, budget {a.budget_s:.0f} s ({_elapsed():.1f} s used)")
-    e = Engine(p, vp, vs)
+    log(f"{a.engine} engine, batch {batch}, {nctx} context(s), {a.steps} timed + {a.warmup} warmup steps, "
+        f"grid {p.nx}x{p.nz} nt={p.nt}, budget {a.budget_s:.0f} s ({_elapsed():.1f} s used)")
+    es = [Engine(p, vp, vs) for _ in range(nctx)]
+    e = es[0]
     tile = None
     if engine == abi.ENGINE_FUSED:
         from ouc_amd import tuning
 
-        if a.tile:
-            cols, pf = (int(v) for v in a.tile.lower().split("x"))
-            e.set_tile(cols, pf)
-            src = "given"
-        else:
-            cols, pf, cached = tuning.tuned(e, batch)
-            src = "cache" if cached else "autotuned"
+        for x in es:
+            x.set_batch_capacity(batch)
+            if a.tile:
+                cols, pf = (int(v) for v in a.tile.lower().split("x"))
+                x.set_tile(cols, pf)
+                src = "given"
+            else:
+                cols, pf, cached = tuning.tuned(x, batch)
+                src = "cache" if cached else "autotuned"
         tile = {"core_cols": cols, "prefetch": pf, "source": src}
         log(f"tile {cols} columns, prefetch {pf} ({src})")
     e.set_profiling(True)
-    # The wall-clock guard keeps the whole command inside its budget: a step
-    # starts only if it, the plain and CPU legs still to come, and a margin fit.
+
+    def run(sl):
+        """Steps over shots[sl]: one batch each; with several contexts the
+        batches run at once, batch j on context j mod nctx.  Returns the
+        timings of all of them."""
+        if engine != abi.ENGINE_FUSED:
+            agg = {}
+            for k0 in range(sl.start, sl.stop, batch):
+                e.rtm_batch(shots[k0:k0 + batch])
+                for key, v in e.timings().items():
+                    agg[key] = agg.get(key, 0) + v
+            return agg
+        rtm_concurrent(es, shots[sl])
+        return e.timings()
+
+    # The wall-clock guard keeps the whole command inside its budget: the
+    # timed region runs only as many steps as fit after the warmup's measured
+    # rate, with the solo, plain and CPU legs still to come and a margin.
     reserve = (0.0 if a.no_plain or engine != abi.ENGINE_FUSED else 25.0) + \
               (0.0 if a.no_cpu else 2 * a.cpu_sample_s + 15.0) + 20.0
     k = 0
-    step_wall = []
-    warm_done = 0
-    for w in range(a.warmup):
-        if w > 0 and _elapsed() + max(step_wall) + reserve > a.budget_s:
-            log(f"budget: warmup stopped after {w} steps")
-            break
+    warm_done, per_step = 0, None
+    if a.warmup:
+        # first round: every context captures its loops
+        w1 = min(a.warmup, nctx)
         t1 = time.perf_counter()
-        e.rtm_batch(shots[k:k + batch])
-        step_wall.append(time.perf_counter() - t1)
-        k += batch
-        warm_done += 1
-        log(f"warmup step {w} done ({step_wall[-1]:.1f} s, {_elapsed():.0f} s used)")
-    tim = []
+        run(slice(k, k + w1 * batch))
+        k += w1 * batch
+        warm_done = w1
+        per_step = (time.perf_counter() - t1) / w1
+        log(f"warmup: {w1} step(s) in {per_step * w1:.1f} s ({_elapsed():.0f} s used)")
+        rest = a.warmup - w1
+        if rest and _elapsed() + per_step * (rest + a.steps) + reserve <= a.budget_s:
+            t1 = time.perf_counter()
+            run(slice(k, k + rest * batch))
+            k += rest * batch
+            warm_done += rest
+            per_step = (time.perf_counter() - t1) / rest
+            log(f"warmup: {rest} more step(s) in {per_step * rest:.1f} s ({_elapsed():.0f} s used)")
+        elif rest:
+            log(f"budget: warmup stopped after {w1} steps")
+    steps_run = a.steps
+    solo_reserve = 0.0 if nctx == 1 else (per_step or 0.0) * nctx * 1.2
+    if per_step:
+        fit = int((a.budget_s - _elapsed() - reserve - solo_reserve) / per_step)
+        if fit < steps_run:
+            steps_run = max(1, fit)
+            log(f"budget: timed region runs {steps_run} of {a.steps} steps")
     t0 = time.perf_counter()
-    for s_ in range(a.steps):
-        if tim and _elapsed() + max(step_wall) + reserve > a.budget_s:
-            log(f"budget: timed loop stopped after {len(tim)} of {a.steps} steps")
-            break
-        t1 = time.perf_counter()
-        e.rtm_batch(shots[k:k + batch])
-        step_wall.append(time.perf_counter() - t1)
-        k += batch
-        tim.append(e.timings())
-        log(f"step {s_}: {tim[-1]['total_ms']:.1f} ms device ({_elapsed():.0f} s used)")
+    tim = run(slice(k, k + steps_run * batch))
     wall = time.perf_counter() - t0
-    steps_run = len(tim)
-    dev_ms = sum(t["total_ms"] for t in tim)
+    k += steps_run * batch
+    log(f"timed: {steps_run} steps in {wall:.1f} s ({_elapsed():.0f} s used)")
     nshots = steps_run * batch
-    shots_per_hour = nshots / (dev_ms / 1e3) * 3600.0
-    cell_updates = sum(t["cell_updates"] for t in tim)
-    canonical = sum(t["canonical_bytes"] for t in tim)
-    own = sum(t["own_bytes"] for t in tim)
-    kms = sum(t["step_kernel_ms"] for t in tim)
-    klaunch = sum(t["step_kernel_launches"] for t in tim)
+    shots_per_hour = nshots / wall * 3600.0
+    cell_updates, canonical, own = tim["cell_updates"], tim["canonical_bytes"], tim["own_bytes"]
 
-    # dominant kernel roofline: algorithmic bytes per launch / mean launch time (HIP events)
+    # dominant kernel roofline: algorithmic bytes per launch / mean launch time
+    # (HIP events on context 0's stream over the timed region)
     per_launch_bytes, kernel_name = bytes_model.dominant_kernel_bytes(p, engine, batch)
-    mean_ms = kms / max(klaunch, 1)
-    achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if klaunch else None
-    roofline = {
-        "bound": "hbm", "kernel": kernel_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-        "bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_ms, "launches_timed": klaunch,
-    }
+
+    def kernel_roofline(t):
+        kms, klaunch = t["step_kernel_ms"], t["step_kernel_launches"]
+        mean_ms = kms / max(klaunch, 1)
+        achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9 if klaunch else None
+        return {"bound": "hbm", "kernel": kernel_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_ms, "launches_timed": klaunch}
+
+    roofline = kernel_roofline(tim)
     traffic = bytes_model.measured_traffic(kernel_name, batch if engine == abi.ENGINE_FUSED else 1)
     if traffic is not None:
         roofline["traffic"] = traffic
-    loops = None
-    if engine == abi.ENGINE_FUSED and steps_run:
+    if nctx > 1:
+        roofline["note"] = (f"context 0's launches in the timed region, where {nctx - 1} other context(s) run "
+                            "their batches on the same GPU at the same time; roofline_solo times it alone")
+    loops = solo = None
+    if engine == abi.ENGINE_FUSED:
         fwd_b, bwd_b = bytes_model.fused_loop_bytes(p, batch, strips == abi.STRIP_FP16)
-        fms = sum(t["forward_ms"] for t in tim) / steps_run
-        bms = sum(t["backward_ms"] for t in tim) / steps_run
-        loops = {
-            "forward": {"bytes": fwd_b, "ms": fms, "achieved": fwd_b / (fms * 1e-3) / 1e9,
-                        "frac": fwd_b / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "backward": {"bytes": bwd_b, "ms": bms, "achieved": bwd_b / (bms * 1e-3) / 1e9,
-                         "frac": bwd_b / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "unit": "GB/s",
-            "note": "algorithmic bytes of every kernel of the loop (one batch) over the loop's wall time",
-        }
+        loops = {"path": {"bytes": (fwd_b + bwd_b) * steps_run, "ms": wall * 1e3,
+                          "achieved": (fwd_b + bwd_b) * steps_run / wall / 1e9,
+                          "frac": (fwd_b + bwd_b) * steps_run / wall / 1e9 / HBM_PEAK_GBS},
+                 "unit": "GB/s",
+                 "note": "algorithmic bytes of every kernel of every timed batch over the timed region's wall time"}
+        if nctx > 1:
+            # one batch on context 0 alone: the kernel and the loops without the other contexts
+            t1 = time.perf_counter()
+            e.rtm_batch(shots[k:k + batch])
+            t = e.timings()
+            solo = kernel_roofline(t)
+            if traffic is not None:
+                solo["traffic"] = traffic
+            solo["batch_wall_s"] = time.perf_counter() - t1
+            solo["shots_per_hour"] = batch / (t["total_ms"] / 1e3) * 3600.0
+            tim_solo = t
+            log(f"solo leg: one batch in {solo['batch_wall_s']:.1f} s ({_elapsed():.0f} s used)")
+        else:
+            tim_solo = tim
+        fms = tim_solo["forward_ms"] / (1 if nctx > 1 else steps_run)
+        bms = tim_solo["backward_ms"] / (1 if nctx > 1 else steps_run)
+        loops["forward"] = {"bytes": fwd_b, "ms": fms, "achieved": fwd_b / (fms * 1e-3) / 1e9,
+                            "frac": fwd_b / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        loops["backward"] = {"bytes": bwd_b, "ms": bms, "achieved": bwd_b / (bms * 1e-3) / 1e9,
+                             "frac": bwd_b / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        loops["loops_note"] = "forward and backward: one batch's loops" + (" on context 0 alone" if nctx > 1 else "")
 
     plain = None
-    e.close()
+    for x in es:
+        x.close()
     if not a.no_plain and engine == abi.ENGINE_FUSED:
         pp, _, _, _ = vm.cfg_rtm_2048(nt=a.nt, engine=abi.ENGINE_PLAIN, strip_precision=abi.STRIP_FP32)
         ep = Engine(pp, vp, vs)
@@ -252,6 +303,7 @@ def rtm_bench(a):
         cpu = cpu_baseline(p, vp, vs, [shots[0], shots[32 % len(shots)]], a.cpu_sample_s)
         log(f"CPU leg done ({_elapsed():.0f} s used)")
 
+    conc = f", {nctx} contexts at once" if nctx > 1 else ""
     line = {
         "metric": "RTM shots/hour at 2048x1024 (batched); Gcell-updates/s; HBM GB/s",
         "value": shots_per_hour,
@@ -259,25 +311,26 @@ def rtm_bench(a):
         "n_gpus": 1,
         "steps": steps_run,
         "warmup": warm_done,
-        "ms_per_step": dev_ms / max(steps_run, 1),
+        "ms_per_step": wall * 1e3 / steps_run,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded layered + 3% random model)",
         "config": {"workload": "batched elastic RTM, 2048x1024 interior, 5 m, N=5, 30-cell CPML, "
-                               f"{p.nt} steps, {batch} shots per step, {a.strips} strips, {a.engine} engine",
-                   "nx": p.nx, "nz": p.nz, "nt": p.nt, "batch": batch, "tile": tile},
+                               f"{p.nt} steps, {batch} shots per step{conc}, {a.strips} strips, {a.engine} engine",
+                   "nx": p.nx, "nz": p.nz, "nt": p.nt, "batch": batch, "contexts": nctx, "tile": tile},
         "steps_requested": a.steps,
         "warmup_requested": a.warmup,
-        "gcell_updates_per_s": cell_updates / (dev_ms / 1e3) / 1e9,
-        "canonical_gbs": canonical / (dev_ms / 1e3) / 1e9,
-        "own_bytes_gbs": own / (dev_ms / 1e3) / 1e9 if own else None,
+        "gcell_updates_per_s": cell_updates / wall / 1e9,
+        "canonical_gbs": canonical / wall / 1e9,
+        "own_bytes_gbs": own / wall / 1e9 if own else None,
         "wall_s": wall,
-        "host_wall_ms_per_step": wall * 1e3 / max(steps_run, 1),
+        "device_ms_per_step": tim["total_ms"] / steps_run,
         "plain": plain,
         "batched_over_plain": (shots_per_hour / plain["shots_per_hour"]) if plain else None,
         "roofline": roofline,
+        "roofline_solo": solo,
         "roofline_loops": loops,
         "cpu_baseline": cpu,
         "command_s": _elapsed(),

@@ -183,6 +183,24 @@ ouc_status ouc_rtm_shot(ouc_ctx *c, const ouc_shot *s, const float *rec_in,
 ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots,
                          float *stack_pp_lap, float *stack_ps_lap, float *per_shot_ms);
 
+/* One job's shots on several contexts of the same grid at once (fused
+ * engine): the shots are cut into batches of one size, batch j runs on
+ * context j mod nctx, and every context works on its own stream from its own
+ * host thread, so the band kernels of one batch run beside the core kernels of
+ * another.  The batch size is what nctx equal shares of the free device
+ * memory hold, within each context's ouc_set_batch_capacity (pinned for the
+ * call).  The contexts must share the device and the parameters (not the
+ * tile); they are meant to hold the same model.  Returns the stack of the
+ * Laplacian images summed in shot order on the host: bit for bit what
+ * ouc_rtm_batch gives for the same shots on one context (shots that fall back
+ * to CPML are added in shot order here, after the others there).
+ * per_shot_ms: optional, each shot's batch completion time from the start of
+ * the call.  Timings of the call go to ctxs[0].  At most 8 contexts.  No
+ * counterpart in the reference, which runs its shots one after another (the
+ * shot loop, P-S-seperating.cu:2108-2401, and its stack, 2403-2443). */
+ouc_status ouc_rtm_concurrent(ouc_ctx *const *ctxs, int32_t nctx, const ouc_shot *shots, int32_t nshots,
+                              float *stack_pp_lap, float *stack_ps_lap, float *per_shot_ms);
+
 /* Batched RTM over shots on several grids at once (fused engine): context i
  * (its own grid and model) runs shots[i][0 .. nshots[i]).  Every kernel of a
  * time step is one launch over the shots of all contexts, whatever their grid

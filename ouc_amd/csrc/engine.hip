@@ -596,6 +596,16 @@ ouc_status ouc_rtm_grouped(ouc_ctx *const *ctxs, int32_t nctx, const ouc_shot *c
     return guarded(ctxs[0], [&] { fused_rtm_group(ctxs, nctx, shots, nshots, stack_pp_lap, stack_ps_lap); });
 }
 
+ouc_status ouc_rtm_concurrent(ouc_ctx *const *ctxs, int32_t nctx, const ouc_shot *shots, int32_t nshots,
+                              float *stack_pp_lap, float *stack_ps_lap, float *per_shot_ms) {
+    if (!ctxs || nctx < 1 || !ctxs[0]) return fail(nullptr, OUC_INVALID_ARG, "need at least one context");
+    if (nshots < 1 || !shots) return fail(ctxs[0], OUC_INVALID_ARG, "need at least one shot");
+    return guarded(ctxs[0], [&] {
+        for (int32_t k = 0; k < nshots; ++k) check_shot(ctxs[0], &shots[k]);
+        fused_rtm_concurrent(ctxs, nctx, shots, nshots, stack_pp_lap, stack_ps_lap, per_shot_ms);
+    });
+}
+
 ouc_status ouc_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int32_t nshots, float *stack_pp_lap,
                          float *stack_ps_lap, float *per_shot_ms) {
     if (!c) return fail(nullptr, OUC_INVALID_ARG, "ctx is NULL");

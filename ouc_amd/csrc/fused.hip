@@ -9,9 +9,14 @@
 // which the reference hands over with read_last_wavefiled2
 // (P-S-seperating.cu:1318-1342).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "engine.hpp"
 #include "fused.hpp"
@@ -954,11 +959,29 @@ std::vector<char> monitor_verdict(ouc_ctx *c, int B) {
 }
 
 // Re-runs the listed shots on the CPML fallback context and adds their
-// Laplacian images to the host stacks; returns its device time.
+// Laplacian images to the host stacks; returns its device time.  With
+// shot_lap, each listed shot runs alone and its images go to its slot there
+// (a one-shot stack is that shot's image).
 double rerun_with_cpml(ouc_ctx *c, const ouc_shot *shots, const std::vector<int> &idx, float *stack_pp,
-                       float *stack_ps, float *per_shot_ms, double t0_ms) {
+                       float *stack_ps, float *per_shot_ms, double t0_ms, float *shot_lap = nullptr) {
     ouc_ctx *fb = fallback_ctx(c);
     const size_t ni = (size_t)c->g.nx * c->g.nz;
+    if (shot_lap) {
+        double t = 0;
+        for (int k : idx) {
+            float *pp = shot_lap + (size_t)k * 2 * ni;
+            float ms = 0;
+            ouc_status st = ouc_rtm_batch(fb, &shots[k], 1, pp, pp + ni, &ms);
+            if (st != OUC_OK) throw Error(st, std::string("CPML fallback: ") + ouc_last_error(fb));
+            for (size_t a = 0; a < ni; ++a) {
+                if (stack_pp) stack_pp[a] += pp[a];
+                if (stack_ps) stack_ps[a] += pp[ni + a];
+            }
+            t += fb->last.total_ms;
+            if (per_shot_ms) per_shot_ms[k] = (float)(t0_ms + t);
+        }
+        return t;
+    }
     std::vector<ouc_shot> sub;
     for (int k : idx) sub.push_back(shots[k]);
     std::vector<float> pp(ni), ps(ni), ms(sub.size());
@@ -974,7 +997,7 @@ double rerun_with_cpml(ouc_ctx *c, const ouc_shot *shots, const std::vector<int>
 }
 
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
-                     float *per_shot_ms) {
+                     float *per_shot_ms, float *shot_lap) {
     const Grid &g = c->g;
     for (int k = 0; k < nshots; ++k) {
         const ouc_shot *sh = &shots[k];
@@ -1009,6 +1032,11 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
             shot_images(c, st, b);
             k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[2].f(), stack.f());
             k_accumulate<<<(unsigned)((ni + 255) / 256), 256, 0, c->stream>>>(ni, c->img[3].f(), stack.f() + ni);
+            if (shot_lap) { // copied (and synchronized) before the next shot's images overwrite img
+                float *dst = shot_lap + (size_t)(b0 + b) * 2 * ni;
+                copy_d2h(c->stream, dst, c->img[2].ptr, ni * sizeof(float));
+                copy_d2h(c->stream, dst + ni, c->img[3].ptr, ni * sizeof(float));
+            }
         }
         OUC_HIP(hipEventRecord(c->ev[3], c->stream));
         OUC_HIP(hipStreamSynchronize(c->stream));
@@ -1029,7 +1057,8 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
     float tot = 0;
     OUC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
     double fb_ms = 0;
-    if (!diverged.empty()) fb_ms = rerun_with_cpml(c, shots, diverged, stack_pp, stack_ps, per_shot_ms, tot);
+    if (!diverged.empty())
+        fb_ms = rerun_with_cpml(c, shots, diverged, stack_pp, stack_ps, per_shot_ms, tot, shot_lap);
     c->last.total_ms = tot + fb_ms;
     c->last.forward_ms = fwd_ms;
     c->last.backward_ms = bwd_ms;
@@ -1046,6 +1075,148 @@ void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack
         throw Error(OUC_NUMERIC_FALLBACK, std::to_string(diverged.size()) +
                                               " shot(s) ran again with CPML after the learned boundary diverged");
     }
+}
+
+// One job's shots on n contexts of one grid at once.  Batch j (shots
+// [j*cap, (j+1)*cap)) runs on context j mod n, each context from its own host
+// thread on its own stream, so one batch's latency-bound band kernels run
+// beside another batch's streaming core kernels.  Every batch hands its shots'
+// Laplacian images to the host, and they are committed in shot order into
+// FP32 stacks: the same additions, in the same order, as ouc_rtm_batch's
+// device stack, so the stacks are bit for bit those of one context (shots
+// that fall back to CPML are added in shot order here, after the others
+// there).  A thread starts batch j only when j < next + 2n, where next is
+// the first uncommitted batch, which bounds the images held on the host; the
+// thread that owns batch next is never held back, so the commits progress.
+void fused_rtm_concurrent(ouc_ctx *const *cs, int n, const ouc_shot *shots, int nshots, float *stack_pp,
+                          float *stack_ps, float *per_shot_ms) {
+    if (n < 1 || n > 8) throw Error(OUC_INVALID_ARG, "1 to 8 contexts");
+    const ouc_params &p0 = cs[0]->p;
+    for (int i = 0; i < n; ++i) {
+        if (!cs[i]) throw Error(OUC_INVALID_ARG, "context " + std::to_string(i) + " is NULL");
+        for (int k = 0; k < i; ++k)
+            if (cs[k] == cs[i]) throw Error(OUC_INVALID_ARG, "a context appears twice");
+        const ouc_params &q = cs[i]->p;
+        if (q.engine != OUC_ENGINE_FUSED) throw Error(OUC_INVALID_ARG, "concurrent RTM runs on the fused engine");
+        if (cs[i]->device != cs[0]->device) throw Error(OUC_INVALID_ARG, "contexts must share the device");
+        if (q.nx != p0.nx || q.nz != p0.nz || q.npml != p0.npml || q.half_order != p0.half_order || q.nt != p0.nt ||
+            q.dx != p0.dx || q.dz != p0.dz || q.dt != p0.dt || q.R != p0.R || q.semantics != p0.semantics ||
+            q.strip_precision != p0.strip_precision || q.boundary != p0.boundary || q.laplacian != p0.laplacian ||
+            q.band != p0.band)
+            throw Error(OUC_INVALID_ARG, "contexts must have the same parameters (context " + std::to_string(i) + ")");
+    }
+    const Grid &g = cs[0]->g;
+    const size_t ni = (size_t)g.nx * g.nz;
+    // One batch size for all: what n equal shares of the free memory (plus
+    // what the contexts already hold) fit, within each context's own cap.
+    size_t freeb = 0, total = 0;
+    OUC_HIP(hipMemGetInfo(&freeb, &total));
+    size_t held = 0;
+    for (int i = 0; i < n; ++i) {
+        const FusedState *st = fs(cs[i]);
+        held += st->mem_s.bytes + st->mem_r.bytes + st->strips32.bytes + st->strips16.bytes + st->in32.bytes +
+                st->e8.bytes + st->keyb.bytes + st->ef.bytes + st->decb.bytes + st->gacc.bytes + st->obs.bytes;
+    }
+    const size_t per = per_shot_bytes(cs[0], p0.strip_precision == OUC_STRIP_FP16);
+    int cap = (int)std::max<size_t>(1, (size_t)((freeb + held) * 0.85) / ((size_t)n * per));
+    for (int i = 0; i < n; ++i)
+        if (cs[i]->batch_cap > 0) cap = std::min(cap, cs[i]->batch_cap);
+    cap = std::min(cap, (nshots + n - 1) / n);
+    const int nb = (nshots + cap - 1) / cap;
+    struct CapPin { // the call's batch size on every context, restored afterwards
+        ouc_ctx *const *cs; int n; std::vector<int> saved;
+        ~CapPin() { for (int i = 0; i < n; ++i) cs[i]->batch_cap = saved[i]; }
+    } pin{cs, n, std::vector<int>(n)};
+    for (int i = 0; i < n; ++i) { pin.saved[i] = cs[i]->batch_cap; cs[i]->batch_cap = cap; }
+
+    std::vector<float> spp(ni, 0.f), sps(ni, 0.f);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::map<int, std::vector<float>> pending; // finished, uncommitted batches' images
+    int next = 0, fallbacks = 0;
+    bool stop = false;
+    ouc_status err = OUC_OK;
+    std::string msg;
+    std::vector<ouc_timings> agg(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&](int i) {
+        ouc_ctx *c = cs[i];
+        for (int j = i; j < nb; j += n) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || j < next + 2 * n; });
+                if (stop) return;
+            }
+            const int b0 = j * cap, B = std::min(cap, nshots - b0);
+            std::vector<float> imgs((size_t)B * 2 * ni);
+            ouc_status st = OUC_OK;
+            std::string what;
+            try {
+                OUC_HIP(hipSetDevice(c->device));
+                fused_rtm_batch(c, shots + b0, B, nullptr, nullptr, nullptr, imgs.data());
+            } catch (const Error &e) {
+                st = e.code;
+                what = e.what();
+            } catch (const std::bad_alloc &) {
+                st = OUC_OOM;
+                what = "host allocation failed";
+            } catch (const std::exception &e) {
+                st = OUC_DEVICE;
+                what = e.what();
+            }
+            const float done_ms =
+                std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            std::lock_guard<std::mutex> lk(mu);
+            if (st == OUC_NUMERIC_FALLBACK) { // valid images; the shots ran again on CPML
+                fallbacks += c->last.fallback_shots;
+            } else if (st != OUC_OK) {
+                if (err == OUC_OK) { err = st; msg = "context " + std::to_string(i) + ": " + what; }
+                stop = true;
+                cv.notify_all();
+                return;
+            }
+            ouc_timings &a = agg[i];
+            a.forward_ms += c->last.forward_ms; a.backward_ms += c->last.backward_ms; a.image_ms += c->last.image_ms;
+            a.step_launches += c->last.step_launches; a.step_kernel_ms += c->last.step_kernel_ms;
+            a.step_kernel_launches += c->last.step_kernel_launches; a.cell_updates += c->last.cell_updates;
+            a.canonical_bytes += c->last.canonical_bytes; a.own_bytes += c->last.own_bytes;
+            a.boundary_ratio = std::max(a.boundary_ratio, c->last.boundary_ratio);
+            if (per_shot_ms)
+                for (int k = 0; k < B; ++k) per_shot_ms[b0 + k] = done_ms;
+            pending.emplace(j, std::move(imgs));
+            for (auto it = pending.find(next); it != pending.end(); it = pending.find(next)) {
+                const int c0 = next * cap, cB = std::min(cap, nshots - c0);
+                for (int k = 0; k < cB; ++k) {
+                    const float *pp = it->second.data() + (size_t)k * 2 * ni, *ps = pp + ni;
+                    for (size_t q = 0; q < ni; ++q) { spp[q] += pp[q]; sps[q] += ps[q]; }
+                }
+                pending.erase(it);
+                ++next;
+            }
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; ++i) th.emplace_back(worker, i);
+    for (auto &t : th) t.join();
+    if (err != OUC_OK) throw Error(err, msg);
+    if (stack_pp) std::memcpy(stack_pp, spp.data(), ni * sizeof(float));
+    if (stack_ps) std::memcpy(stack_ps, sps.data(), ni * sizeof(float));
+    ouc_timings tot{};
+    for (const ouc_timings &a : agg) {
+        tot.forward_ms += a.forward_ms; tot.backward_ms += a.backward_ms; tot.image_ms += a.image_ms;
+        tot.step_launches += a.step_launches; tot.step_kernel_ms += a.step_kernel_ms;
+        tot.step_kernel_launches += a.step_kernel_launches; tot.cell_updates += a.cell_updates;
+        tot.canonical_bytes += a.canonical_bytes; tot.own_bytes += a.own_bytes;
+        tot.boundary_ratio = std::max(tot.boundary_ratio, a.boundary_ratio);
+    }
+    tot.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    tot.fallback_shots = fallbacks;
+    cs[0]->last = tot;
+    if (fallbacks > 0)
+        throw Error(OUC_NUMERIC_FALLBACK,
+                    std::to_string(fallbacks) + " shot(s) ran again with CPML after the learned boundary diverged");
 }
 
 // ---------------------------------------------------------------------------

@@ -12,8 +12,13 @@ void fused_destroy(ouc_ctx *c);
 // One RTM shot through the fused kernels (a batch of one); images to host.
 void fused_rtm_one(ouc_ctx *c, const ouc_shot *s, const float *rec_in, float *img_pp, float *img_ps, float *lap_pp,
                    float *lap_ps);
+// shot_lap (optional, host [nshots][2][nx*nz]) receives each shot's Laplacian
+// PP and PS images.
 void fused_rtm_batch(ouc_ctx *c, const ouc_shot *shots, int nshots, float *stack_pp, float *stack_ps,
-                     float *per_shot_ms);
+                     float *per_shot_ms, float *shot_lap = nullptr);
+// One job's shots on several contexts of one grid at once (ouc_rtm_concurrent).
+void fused_rtm_concurrent(ouc_ctx *const *ctxs, int n, const ouc_shot *shots, int nshots, float *stack_pp,
+                          float *stack_ps, float *per_shot_ms);
 void fused_rtm_group(ouc_ctx *const *ctxs, int n, const ouc_shot *const *shots, const int32_t *nshots,
                      float *const *stack_pp, float *const *stack_ps);
 double fused_own_bytes(const ouc_ctx *c);

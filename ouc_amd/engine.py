@@ -43,6 +43,8 @@ def lib():
     L.ouc_rtm_batch.restype = C.c_int
     L.ouc_rtm_grouped.argtypes = [P(C.c_void_p), C.c_int32, P(P(abi.Shot)), P(C.c_int32), P(fp), P(fp)]
     L.ouc_rtm_grouped.restype = C.c_int
+    L.ouc_rtm_concurrent.argtypes = [P(C.c_void_p), C.c_int32, P(abi.Shot), C.c_int32, fp, fp, fp]
+    L.ouc_rtm_concurrent.restype = C.c_int
     L.ouc_get_timings.argtypes = [C.c_void_p, P(abi.Timings)]
     L.ouc_get_timings.restype = C.c_int
     L.ouc_set_profiling.argtypes = [C.c_void_p, C.c_int32]
@@ -320,6 +322,31 @@ def rtm_grouped(engines, shot_lists):
     ps = (fp * n)(*[abi.fptr(o["stack_ps"]) for o in outs])
     engines[0]._check(lib().ouc_rtm_grouped(hs, n, sp, ns, pp, ps))
     return outs
+
+
+def rtm_concurrent(engines, shots, per_shot=False):
+    """One job's shots on several engines of the same grid at once
+    (ouc_rtm_concurrent): batch j runs on engines[j % n], each engine on its
+    own stream.  Returns dict(stack_pp, stack_ps), bit for bit the stacks of
+    engines[0].rtm_batch(shots); OUC_NUMERIC_FALLBACK sets fell_back on the
+    first engine and warns."""
+    n = len(engines)
+    hs = (C.c_void_p * n)(*[e._h for e in engines])
+    m = len(shots)
+    arr = (abi.Shot * max(1, m))()
+    keep = []
+    for k, sh in enumerate(shots):
+        s, w = _shot_struct(sh, engines[0].p.nt)
+        arr[k] = s
+        keep.append(w)
+    spp = np.zeros((engines[0].p.nx, engines[0].p.nz), np.float32)
+    sps = np.zeros_like(spp)
+    ms = np.zeros(max(1, m), np.float32)
+    engines[0]._check(lib().ouc_rtm_concurrent(hs, n, arr, m, abi.fptr(spp), abi.fptr(sps), abi.fptr(ms)))
+    out = {"stack_pp": spp, "stack_ps": sps}
+    if per_shot:
+        out["per_shot_ms"] = ms[:m]
+    return out
 
 
 def write_image_files(directory, pp_lap, ps_lap, shot):
