@@ -175,7 +175,11 @@ def rtm_bench(a):
                 src = "cache" if cached else "autotuned"
         tile = {"core_cols": cols, "prefetch": pf, "source": src}
         log(f"tile {cols} columns, prefetch {pf} ({src})")
-    e.set_profiling(True)
+    # The dominant kernel is timed with HIP events on context 0's stream: in the
+    # timed region with one context; with several, whose kernels overlap there
+    # (a launch's duration then includes the other contexts' work), in a solo
+    # leg of one batch on context 0 after the timed region.
+    e.set_profiling(nctx == 1)
 
     def run(sl):
         """Steps over shots[sl]: one batch each; with several contexts the
@@ -234,7 +238,7 @@ def rtm_bench(a):
     cell_updates, canonical, own = tim["cell_updates"], tim["canonical_bytes"], tim["own_bytes"]
 
     # dominant kernel roofline: algorithmic bytes per launch / mean launch time
-    # (HIP events on context 0's stream over the timed region)
+    # (HIP events on context 0's stream)
     per_launch_bytes, kernel_name = bytes_model.dominant_kernel_bytes(p, engine, batch)
 
     def kernel_roofline(t):
@@ -245,13 +249,8 @@ def rtm_bench(a):
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                 "bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_ms, "launches_timed": klaunch}
 
-    roofline = kernel_roofline(tim)
+    roofline = kernel_roofline(tim) if nctx == 1 else None
     traffic = bytes_model.measured_traffic(kernel_name, batch if engine == abi.ENGINE_FUSED else 1)
-    if traffic is not None:
-        roofline["traffic"] = traffic
-    if nctx > 1:
-        roofline["note"] = (f"context 0's launches in the timed region, where {nctx - 1} other context(s) run "
-                            "their batches on the same GPU at the same time; roofline_solo times it alone")
     loops = solo = None
     if engine == abi.ENGINE_FUSED:
         fwd_b, bwd_b = bytes_model.fused_loop_bytes(p, batch, strips == abi.STRIP_FP16)
@@ -262,26 +261,29 @@ def rtm_bench(a):
                  "note": "algorithmic bytes of every kernel of every timed batch over the timed region's wall time"}
         if nctx > 1:
             # one batch on context 0 alone: the kernel and the loops without the other contexts
+            e.set_profiling(True)
             t1 = time.perf_counter()
             e.rtm_batch(shots[k:k + batch])
-            t = e.timings()
-            solo = kernel_roofline(t)
-            if traffic is not None:
-                solo["traffic"] = traffic
-            solo["batch_wall_s"] = time.perf_counter() - t1
-            solo["shots_per_hour"] = batch / (t["total_ms"] / 1e3) * 3600.0
-            tim_solo = t
+            tim_solo = e.timings()
+            roofline = kernel_roofline(tim_solo)
+            roofline["measured_on"] = (f"context 0 alone, one batch of {batch} shots after the timed region "
+                                       f"(in the timed region {nctx} contexts overlap on the GPU)")
+            solo = {"batch_wall_s": time.perf_counter() - t1,
+                    "shots_per_hour": batch / (tim_solo["total_ms"] / 1e3) * 3600.0}
             log(f"solo leg: one batch in {solo['batch_wall_s']:.1f} s ({_elapsed():.0f} s used)")
         else:
             tim_solo = tim
-        fms = tim_solo["forward_ms"] / (1 if nctx > 1 else steps_run)
-        bms = tim_solo["backward_ms"] / (1 if nctx > 1 else steps_run)
+        per = 1 if solo else steps_run
+        fms = tim_solo["forward_ms"] / per
+        bms = tim_solo["backward_ms"] / per
         loops["forward"] = {"bytes": fwd_b, "ms": fms, "achieved": fwd_b / (fms * 1e-3) / 1e9,
                             "frac": fwd_b / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         loops["backward"] = {"bytes": bwd_b, "ms": bms, "achieved": bwd_b / (bms * 1e-3) / 1e9,
                              "frac": bwd_b / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        loops["loops_note"] = "forward and backward: one batch's loops" + (" on context 0 alone" if nctx > 1 else "")
+        loops["loops_note"] = "forward and backward: one batch's loops" + (" on context 0 alone" if solo else "")
 
+    if roofline and traffic is not None:
+        roofline["traffic"] = traffic
     plain = None
     for x in es:
         x.close()
@@ -330,7 +332,7 @@ def rtm_bench(a):
         "plain": plain,
         "batched_over_plain": (shots_per_hour / plain["shots_per_hour"]) if plain else None,
         "roofline": roofline,
-        "roofline_solo": solo,
+        "solo_batch": solo,
         "roofline_loops": loops,
         "cpu_baseline": cpu,
         "command_s": _elapsed(),
