@@ -105,3 +105,5 @@ def test_argument_checks_need_no_gpu():
     assert lib().ouc_model_update(0, 0, abi.fptr(v), abi.fptr(v), 30.0, 0, abi.fptr(v)) == abi.OUC_INVALID_ARG
     assert lib().ouc_fwi_gradient(None, None, 1, None, None, None, None, None) == abi.OUC_INVALID_ARG
     assert lib().ouc_rtm_grouped(None, 0, None, None, None, None) == abi.OUC_INVALID_ARG
+    assert lib().ouc_rtm_concurrent(None, 0, None, 0, None, None, None) == abi.OUC_INVALID_ARG
+    assert lib().ouc_rtm_concurrent(None, 3, None, 8, None, None, None) == abi.OUC_INVALID_ARG
