@@ -1148,10 +1148,11 @@ void fused_rtm_concurrent(ouc_ctx *const *cs, int n, const ouc_shot *shots, int 
                 if (stop) return;
             }
             const int b0 = j * cap, B = std::min(cap, nshots - b0);
-            std::vector<float> imgs((size_t)B * 2 * ni);
+            std::vector<float> imgs;
             ouc_status st = OUC_OK;
             std::string what;
             try {
+                imgs.resize((size_t)B * 2 * ni);
                 OUC_HIP(hipSetDevice(c->device));
                 fused_rtm_batch(c, shots + b0, B, nullptr, nullptr, nullptr, imgs.data());
             } catch (const Error &e) {
@@ -1183,7 +1184,14 @@ void fused_rtm_concurrent(ouc_ctx *const *cs, int n, const ouc_shot *shots, int 
             a.boundary_ratio = std::max(a.boundary_ratio, c->last.boundary_ratio);
             if (per_shot_ms)
                 for (int k = 0; k < B; ++k) per_shot_ms[b0 + k] = done_ms;
-            pending.emplace(j, std::move(imgs));
+            try {
+                pending.emplace(j, std::move(imgs));
+            } catch (const std::bad_alloc &) {
+                if (err == OUC_OK) { err = OUC_OOM; msg = "host allocation failed"; }
+                stop = true;
+                cv.notify_all();
+                return;
+            }
             for (auto it = pending.find(next); it != pending.end(); it = pending.find(next)) {
                 const int c0 = next * cap, cB = std::min(cap, nshots - c0);
                 for (int k = 0; k < cB; ++k) {
@@ -1198,7 +1206,17 @@ void fused_rtm_concurrent(ouc_ctx *const *cs, int n, const ouc_shot *shots, int 
     };
     std::vector<std::thread> th;
     th.reserve(n);
-    for (int i = 0; i < n; ++i) th.emplace_back(worker, i);
+    try {
+        for (int i = 0; i < n; ++i) th.emplace_back(worker, i);
+    } catch (...) { // a thread could not start: stop the others before unwinding
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto &t : th) t.join();
+        throw;
+    }
     for (auto &t : th) t.join();
     if (err != OUC_OK) throw Error(err, msg);
     if (stack_pp) std::memcpy(stack_pp, spp.data(), ni * sizeof(float));
