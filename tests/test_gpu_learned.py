@@ -258,8 +258,9 @@ def test_learned_boundary_rtm_batch_runs_and_stacks(LB, weights):
                                            atol=1e-6 * np.abs(singles[0]).max())
         rels[name] = np.linalg.norm(st["stack_pp"] - ref) / np.linalg.norm(ref)
     print("PP stack vs 40-cell CPML, rel L2: " + ", ".join(f"{k} {v:.3f}" for k, v in rels.items()))
-    assert rels["learned"] < 0.3
-    assert rels["learned"] < 0.5 * rels["no_network"]
+    # measured 0.056 (learned) and 0.781 (no network); the run is bitwise reproducible
+    assert rels["learned"] < 0.08
+    assert rels["learned"] < 0.15 * rels["no_network"]
 
 
 @pytest.mark.parametrize("boundary", ["cpml", "learned"])
