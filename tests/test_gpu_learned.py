@@ -166,10 +166,12 @@ def test_trained_network_beats_the_band_without_it(LB, weights):
         rows.append(r)
         print(f"case {k}: vp {c.vp_ext.min():.0f}-{c.vp_ext.max():.0f} m/s, fm {c.fm:.1f} Hz: "
               + ", ".join(f"{a} {b:.1f} dB" for a, b in r.items()))
+    # measured -25.5, -21.1, -21.8 dB; at least 14.5 dB below the band without the network and
+    # 16.5 dB below the CPML profile on the same cells (the runs are bitwise reproducible)
     for r in rows:
-        assert r["learned"] < -12.0
-        assert r["learned"] < r["no_network"] - 8.0
-        assert r["learned"] < r["cpml_band"] - 8.0
+        assert r["learned"] < -18.0
+        assert r["learned"] < r["no_network"] - 12.0
+        assert r["learned"] < r["cpml_band"] - 12.0
 
 
 def test_trained_network_at_the_training_size(LB, weights):
@@ -185,9 +187,10 @@ def test_trained_network_at_the_training_size(LB, weights):
         r = LB.reflection_db(c, weights, variants=("learned", "no_network", "cpml_band", "cpml_40"), stride=16)
         print(f"[512 case {k}] vp {c.vp_ext.min():.0f}-{c.vp_ext.max():.0f} m/s, fm {c.fm:.1f} Hz: "
               + ", ".join(f"{a} {b:.1f} dB" for a, b in r.items()), flush=True)
-        assert r["learned"] < -12.0
-        assert r["learned"] < r["no_network"] - 8.0
-        assert r["learned"] < r["cpml_band"] - 8.0
+        # measured -21.7 and -27.1 dB; at least 20.8 dB below the band without the network
+        assert r["learned"] < -18.0
+        assert r["learned"] < r["no_network"] - 16.0
+        assert r["learned"] < r["cpml_band"] - 16.0
 
 
 def test_long_runs_stay_stable(LB, weights):
