@@ -118,3 +118,26 @@ def test_concurrent_rejects_contexts_that_differ():
     finally:
         a.close()
         b.close()
+
+
+def test_concurrent_at_the_bench_grid():
+    """The bench's grid and tile (2048 x 1024, 30-cell CPML, FP16 strips, core
+    tile 108 x 1) cut to 60 steps: six shots in batches of two over three
+    contexts against one context's batch of six, bit for bit."""
+    from ouc_amd import velocity_models as vm
+
+    print("[case] concurrent: 2048x1024, nt 60, 3 contexts x batches of 2, 6 shots", flush=True)
+    p, vp, vs, shots = vm.cfg_rtm_2048(nt=60, engine=abi.ENGINE_FUSED, strip_precision=abi.STRIP_FP16)
+    shots = shots[:6]
+    with Engine(p, vp, vs) as one:
+        one.set_tile(108, 1)
+        ref = one.rtm_batch(shots)
+    es = [Engine(p, vp, vs) for _ in range(3)]
+    try:
+        for e in es:
+            e.set_batch_capacity(2)
+            e.set_tile(108, 1)
+        _same(rtm_concurrent(es, shots), ref, "2048x1024")
+    finally:
+        for e in es:
+            e.close()
